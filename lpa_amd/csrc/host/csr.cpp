@@ -1,0 +1,197 @@
+// csr.cpp -- CSR construction, BFS relabelling, text edge lists.
+//
+// Replaces File::read_from_edge_list (src/file.cpp:115-195) and
+// CgrCompressor::load_graph (src/compressor.cpp:22-68): both build sorted
+// adjacency lists from a text edge list, single threaded, with 8-byte ids
+// and without removing duplicates.  Here: 32-bit ids, 64-bit offsets,
+// counting sort by source over OpenMP threads, duplicates and self loops
+// removed.
+#include <omp.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <thread>
+
+#include "host_internal.hpp"
+
+namespace lpah {
+
+static thread_local std::string g_err;
+void set_error(const std::string &m) { g_err = m; }
+
+static void exclusive_scan(std::vector<uint64_t> &a) {
+    uint64_t s = 0;
+    for (auto &x : a) { uint64_t t = x; x = s; s += t; }
+}
+
+void build_csr(uint64_t n, std::vector<uint32_t> &src, std::vector<uint32_t> &dst,
+               bool undirected, lpah_csr &out) {
+    const uint64_t m = src.size();
+    if (dst.size() != m) throw Error(E_ARG, "src/dst length mismatch");
+    std::vector<uint64_t> cnt(n + 1, 0);
+#pragma omp parallel for schedule(static)
+    for (uint64_t i = 0; i < m; i++) {
+        uint32_t u = src[i], v = dst[i];
+        if (u >= n || v >= n) continue;   // validated below
+        if (u == v) continue;
+        __atomic_fetch_add(&cnt[u], 1, __ATOMIC_RELAXED);
+        if (undirected) __atomic_fetch_add(&cnt[v], 1, __ATOMIC_RELAXED);
+    }
+    for (uint64_t i = 0; i < m; i++)
+        if (src[i] >= n || dst[i] >= n) throw Error(E_RANGE, "edge endpoint >= n");
+    exclusive_scan(cnt);              // cnt[v] = start of row v, cnt[n] = total
+    std::vector<uint64_t> cur(cnt.begin(), cnt.end() - 1);
+    std::vector<uint32_t> col0(cnt[n]);
+#pragma omp parallel for schedule(static)
+    for (uint64_t i = 0; i < m; i++) {
+        uint32_t u = src[i], v = dst[i];
+        if (u == v) continue;
+        col0[__atomic_fetch_add(&cur[u], 1, __ATOMIC_RELAXED)] = v;
+        if (undirected) col0[__atomic_fetch_add(&cur[v], 1, __ATOMIC_RELAXED)] = u;
+    }
+    std::vector<uint32_t>().swap(src);
+    std::vector<uint32_t>().swap(dst);
+    std::vector<uint64_t>().swap(cur);
+    std::vector<uint64_t> deg(n + 1, 0);
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (uint64_t v = 0; v < n; v++) {
+        auto b = col0.begin() + cnt[v], e = col0.begin() + cnt[v + 1];
+        std::sort(b, e);
+        deg[v] = (uint64_t)(std::unique(b, e) - b);
+    }
+    exclusive_scan(deg);
+    out.n = n;
+    out.col.assign(deg[n], 0);
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (uint64_t v = 0; v < n; v++)
+        std::copy(col0.begin() + cnt[v], col0.begin() + cnt[v] + (deg[v + 1] - deg[v]),
+                  out.col.begin() + deg[v]);
+    out.row.swap(deg);
+}
+
+}  // namespace lpah
+
+using namespace lpah;
+
+extern "C" {
+
+const char *lpah_last_error(void) { return lpah::g_err.c_str(); }
+
+int lpah_threads(void) { return omp_get_max_threads(); }
+
+int lpah_csr_from_edges(uint64_t n, const uint32_t *src, const uint32_t *dst, uint64_t m,
+                        int undirected, lpah_csr **out) {
+    return guard([&] {
+        if (!out || (m && (!src || !dst))) throw Error(E_ARG, "null argument");
+        if (n >= (1ull << 30)) throw Error(E_RANGE, "n must be < 2^30");
+        std::vector<uint32_t> s(src, src + m), d(dst, dst + m);
+        auto *g = new lpah_csr;
+        try { build_csr(n, s, d, undirected != 0, *g); } catch (...) { delete g; throw; }
+        *out = g;
+    });
+}
+
+int lpah_csr_from_arrays(uint64_t n, const uint64_t *row, const uint32_t *col, lpah_csr **out) {
+    return guard([&] {
+        if (!out || !row) throw Error(E_ARG, "null argument");
+        if (row[0] != 0) throw Error(E_FORMAT, "row[0] != 0");
+        for (uint64_t v = 0; v < n; v++) {
+            if (row[v + 1] < row[v]) throw Error(E_FORMAT, "row offsets decrease");
+            for (uint64_t e = row[v]; e < row[v + 1]; e++) {
+                if (col[e] >= n) throw Error(E_RANGE, "neighbour id >= n");
+                if (e > row[v] && col[e] <= col[e - 1]) throw Error(E_FORMAT, "row not strictly ascending");
+            }
+        }
+        auto *g = new lpah_csr;
+        g->n = n;
+        g->row.assign(row, row + n + 1);
+        g->col.assign(col, col + row[n]);
+        *out = g;
+    });
+}
+
+int lpah_csr_read_edge_list(const char *path, int undirected, lpah_csr **out) {
+    return guard([&] {
+        FILE *f = fopen(path, "rb");
+        if (!f) throw Error(E_IO, std::string("cannot open ") + path);
+        std::vector<uint32_t> s, d;
+        std::vector<char> buf(1 << 20);
+        uint64_t val = 0, maxid = 0;
+        bool in_num = false;
+        int field = 0;
+        uint32_t u = 0;
+        size_t got;
+        while ((got = fread(buf.data(), 1, buf.size(), f)) > 0) {
+            for (size_t i = 0; i < got; i++) {
+                char ch = buf[i];
+                if (ch >= '0' && ch <= '9') {
+                    val = val * 10 + (uint64_t)(ch - '0');
+                    if (val >= (1ull << 30)) { fclose(f); throw Error(E_RANGE, "vertex id >= 2^30"); }
+                    in_num = true;
+                } else if (in_num) {
+                    if (field == 0) { u = (uint32_t)val; field = 1; }
+                    else { s.push_back(u); d.push_back((uint32_t)val); field = 0; }
+                    maxid = std::max<uint64_t>(maxid, val);
+                    val = 0; in_num = false;
+                }
+            }
+        }
+        fclose(f);
+        if (in_num && field == 1) { s.push_back(u); d.push_back((uint32_t)val); maxid = std::max<uint64_t>(maxid, val); }
+        uint64_t n = s.empty() ? 0 : maxid + 1;
+        auto *g = new lpah_csr;
+        try { build_csr(n, s, d, undirected != 0, *g); } catch (...) { delete g; throw; }
+        *out = g;
+    });
+}
+
+uint64_t lpah_csr_n(const lpah_csr *g) { return g->n; }
+uint64_t lpah_csr_m(const lpah_csr *g) { return g->col.size(); }
+const uint64_t *lpah_csr_row(const lpah_csr *g) { return g->row.data(); }
+const uint32_t *lpah_csr_col(const lpah_csr *g) { return g->col.data(); }
+uint32_t lpah_csr_max_degree(const lpah_csr *g) {
+    uint64_t mx = 0;
+#pragma omp parallel for reduction(max : mx)
+    for (uint64_t v = 0; v < g->n; v++) mx = std::max(mx, g->row[v + 1] - g->row[v]);
+    return (uint32_t)mx;
+}
+void lpah_csr_free(lpah_csr *g) { delete g; }
+
+int lpah_csr_bfs_relabel(const lpah_csr *g, uint32_t root, uint32_t *perm_out, lpah_csr **out) {
+    return guard([&] {
+        const uint64_t n = g->n;
+        if (n == 0) { *out = new lpah_csr(*g); return; }
+        if (root >= n) throw Error(E_RANGE, "root >= n");
+        const uint32_t NONE = 0xFFFFFFFFu;
+        std::vector<uint32_t> nid(n, NONE), order;
+        order.reserve(n);
+        nid[root] = 0;
+        order.push_back(root);
+        for (uint64_t h = 0; h < order.size(); h++) {
+            uint32_t u = order[h];
+            for (uint64_t e = g->row[u]; e < g->row[u + 1]; e++) {
+                uint32_t w = g->col[e];
+                if (nid[w] == NONE) { nid[w] = (uint32_t)order.size(); order.push_back(w); }
+            }
+        }
+        for (uint64_t v = 0; v < n; v++)
+            if (nid[v] == NONE) { nid[v] = (uint32_t)order.size(); order.push_back((uint32_t)v); }
+        auto *r = new lpah_csr;
+        r->n = n;
+        r->row.assign(n + 1, 0);
+        for (uint64_t i = 0; i < n; i++) r->row[i + 1] = r->row[i] + (g->row[order[i] + 1] - g->row[order[i]]);
+        r->col.resize(g->col.size());
+#pragma omp parallel for schedule(dynamic, 4096)
+        for (uint64_t i = 0; i < n; i++) {
+            uint32_t u = order[i];
+            uint64_t o = r->row[i];
+            for (uint64_t e = g->row[u]; e < g->row[u + 1]; e++) r->col[o++] = nid[g->col[e]];
+            std::sort(r->col.begin() + r->row[i], r->col.begin() + r->row[i + 1]);
+        }
+        if (perm_out) std::copy(nid.begin(), nid.end(), perm_out);
+        *out = r;
+    });
+}
+
+}  // extern "C"
