@@ -1,0 +1,153 @@
+/*
+ * lpa_amd.h -- C ABI of the MI355X (gfx950) label-propagation engine,
+ * liblpa_hip.so.  Plain pointers and sizes; no C++ or torch types.
+ *
+ * The reference's boundary is in-process C++ (the baselines/<method> propagators
+ * launching CUDA kernels).  Each entry point below names the reference
+ * interface it replaces:
+ *
+ *   lpa_engine_create/destroy   init_gmem / free_gmem
+ *                               (baselines/nova/nova_propogator.cuh:313-345, 381-394)
+ *   lpa_graph_upload            transfer_data, the H2D copy of the CGR words,
+ *                               offsets, vertex order and small CSR
+ *                               (nova_propogator.cuh:347-374)
+ *   lpa_graph_degrees           the CSR degrees the binning reads
+ *                               (utils.h:44-85 reads offsets[v+1]-offsets[v])
+ *   lpa_schedule_degree         tiny_node_assignment + binary_search binning
+ *                               (nova_propogator.cuh:210-311; utils.h:44-85)
+ *   lpa_schedule_bins           (no counterpart) explicit per-vertex bins, used
+ *                               by the learned schedule
+ *   lpa_lp_config               compile-time CMS / Decoder parameters of the
+ *                               reference (cms.cuh:7; nova/kernel.cuh:28)
+ *   lpa_labels_init             initialize_labels (nova/kernel.cuh:57-64)
+ *   lpa_labels_get              the D2H copy of labels (nova_propogator.cuh:123)
+ *   lpa_nova_iterate            NovaPlusPropagator::iterate: propagate_large_vertex,
+ *                               propagate_mid_vertex, l_small_*, l_tiny_*
+ *                               (baselines/novaplus/novaplus_propogator.cuh:131-199;
+ *                                baselines/novaplus/kernel.cuh:14-191)
+ *   lpa_cglp_decode             decompress_large_vertex / decompress_mid_vertex
+ *                               (baselines/cglp/kernel.cuh:10-39)
+ *   lpa_cglp_iterate            CGLPPropagator::iterate: decode, then the
+ *                               gather/sort/scan/compute_max_labels LP stage
+ *                               (baselines/cglp/cglp_propagator.cuh:205-335)
+ *   lpa_cglp_csr_get            (no counterpart) copy of the decoded CSR, for checks
+ *   lpa_time_iterations         the per-iteration Timer around iterate
+ *                               (nova_propogator.cuh:114-125), with HIP events
+ *
+ * Semantics ("LP-SC"): synchronous label propagation with double-buffered
+ * labels; a vertex takes the most frequent label among its neighbours, ties
+ * to the smallest label; no (labelled) neighbour -> keeps its label.  With
+ * CMS enabled, vertices of degree > cms_thresh rank labels by a count-min
+ * estimate (depth d, width w, multiply-shift hashes from splitmix64(seed)).
+ * Semi-supervised mode: 0xFFFFFFFF = unlabelled (ignored), initially
+ * labelled entries stay clamped.  Labels must be < 2^31 (or 0xFFFFFFFF).
+ *
+ * Conventions: every function returns LPA_OK (0) or a negative status and
+ * leaves a message for lpa_last_error() (thread-local).  Device memory is
+ * owned by the engine handle.  One host thread per engine; all work is
+ * queued on the engine's own stream (default priority).
+ */
+#ifndef LPA_AMD_H
+#define LPA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    LPA_OK = 0,
+    LPA_ERR_ARG = -1,       /* bad argument */
+    LPA_ERR_HIP = -2,       /* HIP runtime error */
+    LPA_ERR_FORMAT = -3,    /* malformed CGR stream or offsets */
+    LPA_ERR_STATE = -4,     /* call out of order (no graph, no labels, ...) */
+    LPA_ERR_RANGE = -5      /* size or value out of the supported range */
+};
+
+typedef struct lpa_engine lpa_engine;
+
+/* kernel families a vertex can be assigned to */
+enum {
+    LPA_BIN_SKIP = 0,       /* degree 0: label never changes */
+    LPA_BIN_LANE = 1,       /* one lane per vertex, register sort (deg <= 32) */
+    LPA_BIN_WAVE = 2,       /* one wave per vertex, LDS hash */
+    LPA_BIN_BLOCK = 3,      /* one workgroup per vertex, LDS hash */
+    LPA_BIN_GLOBAL = 4,     /* one workgroup per vertex, hash table in HBM */
+    LPA_BIN_CMS = 5,        /* one workgroup per vertex, count-min sketch (forced by degree) */
+    LPA_NBIN_KINDS = 6
+};
+
+typedef struct {
+    uint32_t lane_max;      /* deg <= lane_max -> LANE (<= 32) */
+    uint32_t wave_max;      /* deg <= wave_max -> WAVE */
+    uint32_t block_max;     /* deg <= block_max -> BLOCK, else GLOBAL */
+    uint32_t relayout;      /* 1: reorder records on device so each bin is contiguous */
+} lpa_sched_params;
+
+const char *lpa_last_error(void);
+int lpa_device_count(int *count);
+
+int lpa_engine_create(int device, lpa_engine **out);
+int lpa_engine_destroy(lpa_engine *e);
+
+/* Graph in the reference's File layout: words are the big-endian 32-bit
+ * words of File::graph (nwords of them), offsets the n_c+1 START bit offsets
+ * (File::offset, 64-bit), vertices the n ids (stream order, then small),
+ * small_offsets/small_nbrs the CSR of the n-n_c small vertices (may be
+ * NULL when n_c == n).  The stream is validated on the device: every record
+ * must decode within its offsets to neighbour ids < n. */
+int lpa_graph_upload(lpa_engine *e, uint64_t n, uint64_t n_c,
+                     const uint32_t *words, uint64_t nwords, const uint64_t *offsets,
+                     const uint32_t *vertices, const uint64_t *small_offsets,
+                     const uint32_t *small_nbrs);
+/* deg[v] for every vertex id v (n entries), computed by decoding. */
+int lpa_graph_degrees(lpa_engine *e, uint32_t *deg);
+/* Per-vertex work features for the cost model, by vertex id, 8 floats each:
+ * degree, #intervals, #residuals, encoded bits, #interval segments,
+ * #residual segments, mean run length, log2(1+degree). */
+int lpa_graph_features(lpa_engine *e, float *feat);
+
+int lpa_lp_config(lpa_engine *e, uint32_t K, int semi, uint32_t cms_thresh,
+                  uint32_t cms_depth, uint32_t cms_width, uint64_t cms_seed);
+/* Degree-threshold schedule; CMS vertices (deg > cms_thresh) always go to
+ * LPA_BIN_CMS whatever the schedule says. */
+int lpa_schedule_degree(lpa_engine *e, const lpa_sched_params *p);
+/* Explicit bins by vertex id (LPA_BIN_*); a bin whose kernel cannot hold a
+ * vertex's degree is promoted to the next one that can. */
+int lpa_schedule_bins(lpa_engine *e, const uint8_t *bins, uint32_t relayout);
+/* Vertices per bin kind after scheduling (LPA_NBIN_KINDS entries). */
+int lpa_schedule_counts(lpa_engine *e, uint64_t *counts);
+/* Per bin kind, 4 words: vertices, arcs, CGR stream bits, arcs read from
+ * the small-vertex CSR (4*LPA_NBIN_KINDS entries); the algorithmic bytes of
+ * each kernel family follow from these. */
+int lpa_schedule_stats(lpa_engine *e, uint64_t *stats);
+
+/* labels: n*K entries, layout labels[v*K + k]; NULL -> L0(v) = v for every k.
+ * In semi mode entries != 0xFFFFFFFF are clamped seeds. */
+int lpa_labels_init(lpa_engine *e, const uint32_t *labels);
+/* Current labels (n*K). */
+int lpa_labels_get(lpa_engine *e, uint32_t *labels);
+
+/* One synchronous fused decode+LP step over every bin. */
+int lpa_nova_iterate(lpa_engine *e, uint64_t *changed);
+/* Decode the compressed part to a CSR on the device (CGLP kernel 1). */
+int lpa_cglp_decode(lpa_engine *e);
+/* One synchronous step the two-kernel way: decode to CSR, then LP over it. */
+int lpa_cglp_iterate(lpa_engine *e, uint64_t *changed);
+/* Copy of the device CSR by vertex id: row (n+1), col (m). */
+int lpa_cglp_csr_get(lpa_engine *e, uint64_t *row, uint32_t *col, uint64_t *m);
+
+/* Runs iters steps (method 0 = NOVA fused, 1 = CGLP) and reports the
+ * HIP-event time of each (ms) and the changed counts; captures the step in
+ * a HIP graph when use_graph != 0. */
+int lpa_time_iterations(lpa_engine *e, int method, int iters, int use_graph,
+                        float *ms, uint64_t *changed);
+/* Average duration (ms) of the dominant kernel of the last timed run and
+ * its name (HIP events on the engine stream). */
+int lpa_last_kernel_times(lpa_engine *e, float *ms_per_kind /* LPA_NBIN_KINDS */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,862 @@
+// engine.hip -- liblpa_hip.so: the C ABI of include/lpa_amd.h over the
+// gfx950 kernels of lp_kernels.hpp / prep_kernels.hpp.
+//
+// Device layout after scheduling (all HBM-resident, 64-bit offsets):
+//   words   CGR stream as 64-bit words, records reordered so every bin is a
+//           contiguous run of positions, sorted by degree (descending)
+//           inside the bin; 8 zero words of padding for the readers'
+//           look-ahead
+//   offs    n_c+1 start bit offsets of the reordered records
+//   vid     vertex id of each position
+//   srow/snbr/svid   small-vertex CSR (deg < small threshold), same order rule
+//   L[2]    two label buffers, n*K uint32 each, layout [v][K]
+//   crow/ccol        CGLP: the compressed part decoded to CSR (positions order)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "lp_kernels.hpp"
+#include "lpa_amd.h"
+#include "prep_kernels.hpp"
+
+using namespace lpa;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Fail : std::runtime_error {
+    int code;
+    Fail(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess)                                                                  \
+            throw Fail(LPA_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+template <class F> int guard(F &&f) {
+    try {
+        f();
+        return LPA_OK;
+    } catch (const Fail &e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_err = "out of host memory";
+        return LPA_ERR_RANGE;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return LPA_ERR_ARG;
+    }
+}
+
+template <class T> struct DevBuf {
+    T *p = nullptr;
+    uint64_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(uint64_t cnt) {
+        release();
+        if (cnt == 0) cnt = 1;
+        HIPCHK(hipMalloc((void **)&p, cnt * sizeof(T)));
+        n = cnt;
+    }
+    void upload(const T *h, uint64_t cnt, hipStream_t s) {
+        alloc(cnt);
+        if (cnt) HIPCHK(hipMemcpyAsync(p, h, cnt * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+};
+
+struct Bin {
+    int kind;          // LPA_BIN_*
+    int param;         // LANE: network size N; WAVE/BLOCK: log2 table slots
+    bool csr;          // small-vertex CSR part
+    uint32_t begin, end;
+    uint32_t maxdeg;
+};
+
+inline uint32_t ceil_log2(uint64_t x) {
+    uint32_t l = 0;
+    while ((1ull << l) < x) l++;
+    return l;
+}
+
+uint64_t splitmix64(uint64_t &s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+}  // namespace
+
+struct lpa_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 0;
+    size_t lds_max = 65536;
+
+    // graph as uploaded (file order)
+    uint64_t n = 0, n_c = 0, n_s = 0, nbits = 0, m_c = 0, m_s = 0;
+    DevBuf<uint64_t> words0, offs0;
+    DevBuf<uint32_t> vid0;
+    std::vector<uint64_t> h_offs;           // file-order start offsets
+    std::vector<uint32_t> h_vertices;       // file order
+    std::vector<uint64_t> h_soffs;          // small CSR (file order)
+    std::vector<uint32_t> h_snbr;
+    std::vector<uint32_t> deg_c;            // degree per file-order compressed position
+    DevBuf<float> feat0;                    // features per file-order compressed position
+    bool have_graph = false;
+
+    // scheduled device layout
+    DevBuf<uint64_t> words, offs, srow;
+    DevBuf<uint32_t> vid, snbr, svid;
+    std::vector<Bin> bins;
+    std::vector<uint32_t> order_c;          // device position -> file-order position
+    std::vector<uint32_t> order_s;          // same for the small-vertex CSR
+    uint64_t bin_counts[LPA_NBIN_KINDS] = {0};
+    bool scheduled = false;
+    DevBuf<uint64_t> gtab;
+    uint64_t gcap = 0;
+    uint32_t gblocks = 0;
+
+    // LP state
+    uint32_t K = 1;
+    int semi = 0;
+    uint32_t cms_thresh = 0, cms_depth = 4, cms_width = 1024;
+    uint64_t cms_seed = 0;
+    uint64_t cms_ab[16] = {0};
+    DevBuf<uint32_t> L[2];
+    int cur = 0;
+    bool have_labels = false;
+    DevBuf<unsigned long long> counter;
+    unsigned long long *h_counter = nullptr;   // pinned
+
+    // CGLP
+    DevBuf<uint64_t> crow;
+    DevBuf<uint32_t> ccol;
+    bool have_csr = false;
+
+    // timing of the last timed run, per bin kind
+    float kind_ms[LPA_NBIN_KINDS] = {0};
+
+    ~lpa_engine() {
+        if (h_counter) (void)hipHostFree(h_counter);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void sync() { HIPCHK(hipStreamSynchronize(stream)); }
+
+    StepArgs base_args(bool cglp) const {
+        StepArgs a{};
+        a.words = words.p;
+        a.offs = offs.p;
+        a.vid = vid.p;
+        a.row = srow.p;
+        a.nbr = snbr.p;
+        a.cvid = svid.p;
+        if (cglp) {
+            a.row = crow.p;
+            a.nbr = ccol.p;
+            a.cvid = vid.p;
+        }
+        a.lin = L[cur].p;
+        a.lout = L[cur ^ 1].p;
+        a.K = K;
+        a.cms_depth = cms_depth;
+        a.cms_logw = ceil_log2(cms_width);
+        std::memcpy(a.cms_ab, cms_ab, sizeof cms_ab);
+        a.gtab = gtab.p;
+        a.gcap = gcap;
+        a.changed = counter.p;
+        return a;
+    }
+
+    void launch_bin(const Bin &b, StepArgs a, bool as_csr);
+    void step(bool cglp, hipEvent_t *kind_events);
+};
+
+// ---------------------------------------------------------------- launches
+
+template <bool CSR> static void launch_lane(int N, StepArgs a, uint32_t count, hipStream_t s) {
+    dim3 grid((count + BLOCK - 1) / BLOCK);
+    switch (N) {
+    case 4: hipLaunchKernelGGL((k_lane<4, CSR>), grid, dim3(BLOCK), 0, s, a); break;
+    case 8: hipLaunchKernelGGL((k_lane<8, CSR>), grid, dim3(BLOCK), 0, s, a); break;
+    case 16: hipLaunchKernelGGL((k_lane<16, CSR>), grid, dim3(BLOCK), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_lane<32, CSR>), grid, dim3(BLOCK), 0, s, a); break;
+    }
+}
+
+void lpa_engine::launch_bin(const Bin &b, StepArgs a, bool as_csr) {
+    a.begin = b.begin;
+    a.end = b.end;
+    const uint32_t count = b.end - b.begin;
+    if (count == 0) return;
+    switch (b.kind) {
+    case LPA_BIN_LANE:
+        if (as_csr) launch_lane<true>(b.param, a, count, stream);
+        else launch_lane<false>(b.param, a, count, stream);
+        break;
+    case LPA_BIN_WAVE: {
+        const uint32_t logT = (uint32_t)b.param;
+        const size_t lds = (size_t)(BLOCK / 64) * (8ull << logT);
+        dim3 grid((count + 3) / 4);
+        if (as_csr) hipLaunchKernelGGL((k_group<64, MODE_LDS, true>), grid, dim3(BLOCK), lds, stream, a, logT);
+        else hipLaunchKernelGGL((k_group<64, MODE_LDS, false>), grid, dim3(BLOCK), lds, stream, a, logT);
+        break;
+    }
+    case LPA_BIN_BLOCK: {
+        const uint32_t logT = (uint32_t)b.param;
+        const size_t lds = 8ull << logT;
+        dim3 grid(count);
+        if (as_csr) hipLaunchKernelGGL((k_group<BLOCK, MODE_LDS, true>), grid, dim3(BLOCK), lds, stream, a, logT);
+        else hipLaunchKernelGGL((k_group<BLOCK, MODE_LDS, false>), grid, dim3(BLOCK), lds, stream, a, logT);
+        break;
+    }
+    case LPA_BIN_GLOBAL: {
+        const uint32_t logT = ceil_log2(2ull * b.maxdeg);
+        dim3 grid(std::min<uint32_t>(count, gblocks));
+        if (as_csr) hipLaunchKernelGGL((k_group<BLOCK, MODE_GLOBAL, true>), grid, dim3(BLOCK), 0, stream, a, logT);
+        else hipLaunchKernelGGL((k_group<BLOCK, MODE_GLOBAL, false>), grid, dim3(BLOCK), 0, stream, a, logT);
+        break;
+    }
+    case LPA_BIN_CMS: {
+        const size_t lds = (size_t)cms_depth * cms_width * sizeof(uint32_t);
+        dim3 grid(count);
+        if (as_csr) hipLaunchKernelGGL((k_group<BLOCK, MODE_CMS, true>), grid, dim3(BLOCK), lds, stream, a, 0u);
+        else hipLaunchKernelGGL((k_group<BLOCK, MODE_CMS, false>), grid, dim3(BLOCK), lds, stream, a, 0u);
+        break;
+    }
+    default: break;
+    }
+}
+
+// One synchronous step: clear the counter, every bin reads L[cur] and writes
+// L[cur^1].  Vertices without neighbours are never written (both buffers
+// hold their initial label).
+void lpa_engine::step(bool cglp, hipEvent_t *kind_events) {
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, stream, counter.p, (uint64_t)1, 0ull);
+    StepArgs a = base_args(cglp);
+    bool started[LPA_NBIN_KINDS] = {false};
+    for (const Bin &b : bins) {
+        if (b.kind == LPA_BIN_SKIP || b.end == b.begin) continue;
+        if (kind_events && !started[b.kind]) {
+            HIPCHK(hipEventRecord(kind_events[2 * b.kind], stream));
+            started[b.kind] = true;
+        }
+        launch_bin(b, a, b.csr || cglp);
+        if (kind_events) HIPCHK(hipEventRecord(kind_events[2 * b.kind + 1], stream));
+    }
+    HIPCHK(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ C ABI
+
+static uint64_t lds_budget(const lpa_engine *e);
+
+extern "C" {
+
+const char *lpa_last_error(void) { return g_err.c_str(); }
+
+int lpa_device_count(int *count) {
+    return guard([&] {
+        if (!count) throw Fail(LPA_ERR_ARG, "null argument");
+        HIPCHK(hipGetDeviceCount(count));
+    });
+}
+
+int lpa_engine_create(int device, lpa_engine **out) {
+    return guard([&] {
+        if (!out) throw Fail(LPA_ERR_ARG, "null argument");
+        auto *e = new lpa_engine;
+        try {
+            e->device = device;
+            HIPCHK(hipSetDevice(device));
+            hipDeviceProp_t prop;
+            HIPCHK(hipGetDeviceProperties(&prop, device));
+            e->num_cu = prop.multiProcessorCount;
+            e->lds_max = prop.sharedMemPerBlock;
+            HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+            e->counter.alloc(1);
+            HIPCHK(hipHostMalloc((void **)&e->h_counter, sizeof(unsigned long long), hipHostMallocDefault));
+            e->gblocks = (uint32_t)std::max(1, e->num_cu * 2);
+            const int dyn = (int)(e->lds_max - 8192);
+            (void)hipFuncSetAttribute((const void *)k_group<64, MODE_LDS, false>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+            (void)hipFuncSetAttribute((const void *)k_group<64, MODE_LDS, true>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+            (void)hipFuncSetAttribute((const void *)k_group<BLOCK, MODE_LDS, false>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+            (void)hipFuncSetAttribute((const void *)k_group<BLOCK, MODE_LDS, true>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+            (void)hipFuncSetAttribute((const void *)k_group<BLOCK, MODE_CMS, false>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+            (void)hipFuncSetAttribute((const void *)k_group<BLOCK, MODE_CMS, true>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+            (void)hipGetLastError();
+        } catch (...) {
+            delete e;
+            throw;
+        }
+        *out = e;
+    });
+}
+
+int lpa_engine_destroy(lpa_engine *e) {
+    return guard([&] {
+        if (!e) return;
+        (void)hipSetDevice(e->device);
+        (void)hipStreamSynchronize(e->stream);
+        delete e;
+    });
+}
+
+int lpa_graph_upload(lpa_engine *e, uint64_t n, uint64_t n_c, const uint32_t *words, uint64_t nwords,
+                     const uint64_t *offsets, const uint32_t *vertices, const uint64_t *small_offsets,
+                     const uint32_t *small_nbrs) {
+    return guard([&] {
+        if (!e || !offsets || !vertices || (nwords && !words)) throw Fail(LPA_ERR_ARG, "null argument");
+        if (n >= (1ull << 30)) throw Fail(LPA_ERR_RANGE, "n must be < 2^30");
+        if (n_c > n) throw Fail(LPA_ERR_ARG, "n_c > n");
+        if (n_c >= (1ull << 32) - 1) throw Fail(LPA_ERR_RANGE, "too many compressed vertices");
+        HIPCHK(hipSetDevice(e->device));
+        e->have_graph = e->scheduled = e->have_labels = e->have_csr = false;
+        if (offsets[0] != 0) throw Fail(LPA_ERR_FORMAT, "offsets[0] != 0");
+        for (uint64_t i = 0; i < n_c; i++)
+            if (offsets[i + 1] < offsets[i]) throw Fail(LPA_ERR_FORMAT, "offsets decrease");
+        const uint64_t nbits = offsets[n_c];
+        if ((nbits + 31) / 32 > nwords) throw Fail(LPA_ERR_FORMAT, "stream shorter than the offsets");
+        std::vector<uint8_t> seen(n, 0);
+        for (uint64_t i = 0; i < n; i++) {
+            if (vertices[i] >= n || seen[vertices[i]]) throw Fail(LPA_ERR_FORMAT, "vertices is not a permutation of 0..n-1");
+            seen[vertices[i]] = 1;
+        }
+        const uint64_t n_s = n - n_c;
+        e->h_soffs.assign(n_s + 1, 0);
+        e->h_snbr.clear();
+        if (n_s) {
+            if (!small_offsets) throw Fail(LPA_ERR_ARG, "small vertices present but no small CSR");
+            if (small_offsets[0] != 0) throw Fail(LPA_ERR_FORMAT, "small_offsets[0] != 0");
+            for (uint64_t i = 0; i < n_s; i++)
+                if (small_offsets[i + 1] < small_offsets[i]) throw Fail(LPA_ERR_FORMAT, "small offsets decrease");
+            const uint64_t ms = small_offsets[n_s];
+            if (ms && !small_nbrs) throw Fail(LPA_ERR_ARG, "small_nbrs is null");
+            for (uint64_t i = 0; i < ms; i++)
+                if (small_nbrs[i] >= n) throw Fail(LPA_ERR_FORMAT, "small neighbour id >= n");
+            e->h_soffs.assign(small_offsets, small_offsets + n_s + 1);
+            e->h_snbr.assign(small_nbrs, small_nbrs + ms);
+        }
+        e->n = n;
+        e->n_c = n_c;
+        e->n_s = n_s;
+        e->nbits = nbits;
+        e->m_s = e->h_snbr.size();
+        e->h_offs.assign(offsets, offsets + n_c + 1);
+        e->h_vertices.assign(vertices, vertices + n);
+
+        const uint64_t n64 = (nbits + 63) / 64 + 8;
+        DevBuf<uint32_t> w32;
+        w32.upload(words, nwords, e->stream);
+        e->words0.alloc(n64);
+        hipLaunchKernelGGL(k_pack64, dim3((unsigned)((n64 + 255) / 256)), dim3(256), 0, e->stream, w32.p,
+                           std::min<uint64_t>(nwords, (nbits + 31) / 32), e->words0.p, n64);
+        e->offs0.upload(offsets, n_c + 1, e->stream);
+        e->vid0.upload(vertices, n_c, e->stream);
+        DevBuf<uint32_t> ddeg;
+        ddeg.alloc(n_c);
+        e->feat0.alloc(n_c * NFEAT);
+        e->counter.alloc(1);
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, e->stream, e->counter.p, (uint64_t)1, 0ull);
+        if (n_c)
+            hipLaunchKernelGGL(k_scan_records, dim3((unsigned)((n_c + 255) / 256)), dim3(256), 0, e->stream,
+                               e->words0.p, e->offs0.p, e->vid0.p, n_c, (uint32_t)n, ddeg.p, e->feat0.p,
+                               e->counter.p);
+        HIPCHK(hipGetLastError());
+        unsigned long long bad = 0;
+        HIPCHK(hipMemcpyAsync(&bad, e->counter.p, sizeof bad, hipMemcpyDeviceToHost, e->stream));
+        e->deg_c.assign(n_c, 0);
+        if (n_c) HIPCHK(hipMemcpyAsync(e->deg_c.data(), ddeg.p, n_c * 4, hipMemcpyDeviceToHost, e->stream));
+        e->sync();
+        if (bad) throw Fail(LPA_ERR_FORMAT, "malformed CGR record at stream position " + std::to_string(bad - 1));
+        e->m_c = 0;
+        for (uint32_t d : e->deg_c) e->m_c += d;
+        e->have_graph = true;
+    });
+}
+
+int lpa_graph_degrees(lpa_engine *e, uint32_t *deg) {
+    return guard([&] {
+        if (!e || !deg) throw Fail(LPA_ERR_ARG, "null argument");
+        if (!e->have_graph) throw Fail(LPA_ERR_STATE, "no graph uploaded");
+        for (uint64_t i = 0; i < e->n_c; i++) deg[e->h_vertices[i]] = e->deg_c[i];
+        for (uint64_t j = 0; j < e->n_s; j++)
+            deg[e->h_vertices[e->n_c + j]] = (uint32_t)(e->h_soffs[j + 1] - e->h_soffs[j]);
+    });
+}
+
+int lpa_graph_features(lpa_engine *e, float *feat) {
+    return guard([&] {
+        if (!e || !feat) throw Fail(LPA_ERR_ARG, "null argument");
+        if (!e->have_graph) throw Fail(LPA_ERR_STATE, "no graph uploaded");
+        HIPCHK(hipSetDevice(e->device));
+        std::vector<float> f(e->n_c * NFEAT);
+        if (e->n_c) HIPCHK(hipMemcpy(f.data(), e->feat0.p, f.size() * sizeof(float), hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i < e->n_c; i++)
+            std::memcpy(feat + (uint64_t)e->h_vertices[i] * NFEAT, &f[i * NFEAT], NFEAT * sizeof(float));
+        for (uint64_t j = 0; j < e->n_s; j++) {
+            float *o = feat + (uint64_t)e->h_vertices[e->n_c + j] * NFEAT;
+            const float d = (float)(e->h_soffs[j + 1] - e->h_soffs[j]);
+            const float z[NFEAT] = {d, 0, d, 32.0f * d, 0, 0, 0, std::log2(1.0f + d)};
+            std::memcpy(o, z, sizeof z);
+        }
+    });
+}
+
+int lpa_lp_config(lpa_engine *e, uint32_t K, int semi, uint32_t cms_thresh, uint32_t cms_depth,
+                  uint32_t cms_width, uint64_t cms_seed) {
+    return guard([&] {
+        if (!e) throw Fail(LPA_ERR_ARG, "null argument");
+        if (K == 0 || K > 1024) throw Fail(LPA_ERR_RANGE, "K must be in 1..1024");
+        if (cms_thresh) {
+            if (cms_depth == 0 || cms_depth > 8) throw Fail(LPA_ERR_RANGE, "cms_depth must be in 1..8");
+            if (cms_width == 0 || (cms_width & (cms_width - 1))) throw Fail(LPA_ERR_RANGE, "cms_width must be a power of two");
+            if ((uint64_t)cms_depth * cms_width * 4 > lds_budget(e)) throw Fail(LPA_ERR_RANGE, "CMS sketch exceeds LDS");
+        }
+        const bool changed_cms = (cms_thresh != e->cms_thresh);
+        e->K = K;
+        e->semi = semi ? 1 : 0;
+        e->cms_thresh = cms_thresh;
+        e->cms_depth = cms_depth ? cms_depth : 4;
+        e->cms_width = cms_width ? cms_width : 1024;
+        e->cms_seed = cms_seed;
+        uint64_t s = cms_seed;
+        std::memset(e->cms_ab, 0, sizeof e->cms_ab);
+        for (uint32_t r = 0; r < e->cms_depth && r < 8; r++) {
+            e->cms_ab[2 * r] = splitmix64(s) | 1ull;
+            e->cms_ab[2 * r + 1] = splitmix64(s);
+        }
+        e->have_labels = false;
+        if (changed_cms) e->scheduled = false;   // CMS membership is part of the schedule
+    });
+}
+
+// Builds the device layout from a kind per file-order position (compressed
+// positions first, then small positions).
+static void build_layout(lpa_engine *e, const std::vector<uint8_t> &kind_c, const std::vector<uint8_t> &kind_s,
+                         const lpa_sched_params &p) {
+    auto lane_n = [](uint32_t d) { return d <= 4 ? 4 : d <= 8 ? 8 : d <= 16 ? 16 : 32; };
+    auto sub_of = [&](int kind, uint32_t d) -> int {
+        switch (kind) {
+        case LPA_BIN_LANE: return lane_n(d);
+        case LPA_BIN_WAVE: return (int)std::max<uint32_t>(6, ceil_log2(2ull * std::max<uint32_t>(d, 1)));
+        case LPA_BIN_BLOCK: return (int)std::max<uint32_t>(8, ceil_log2(2ull * std::max<uint32_t>(d, 1)));
+        default: return 0;
+        }
+    };
+    e->bins.clear();
+    std::memset(e->bin_counts, 0, sizeof e->bin_counts);
+    uint64_t gmax = 0;
+    for (int part = 0; part < 2; part++) {
+        const bool csr = part == 1;
+        const uint64_t cnt = csr ? e->n_s : e->n_c;
+        const auto &kinds = csr ? kind_s : kind_c;
+        auto deg = [&](uint64_t i) -> uint32_t {
+            return csr ? (uint32_t)(e->h_soffs[i + 1] - e->h_soffs[i]) : e->deg_c[i];
+        };
+        std::vector<uint32_t> order(cnt);
+        std::iota(order.begin(), order.end(), 0u);
+        std::vector<int> sub(cnt);
+        for (uint64_t i = 0; i < cnt; i++) sub[i] = sub_of(kinds[i], deg(i));
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+            if (kinds[x] != kinds[y]) return kinds[x] < kinds[y];
+            if (sub[x] != sub[y]) return sub[x] < sub[y];
+            return deg(x) > deg(y);
+        });
+        for (uint64_t q = 0; q < cnt;) {
+            uint64_t r = q;
+            const uint32_t x = order[q];
+            while (r < cnt && kinds[order[r]] == kinds[x] && sub[order[r]] == sub[x]) r++;
+            Bin b{kinds[x], sub[x], csr, (uint32_t)q, (uint32_t)r, deg(x)};
+            e->bins.push_back(b);
+            e->bin_counts[b.kind] += r - q;
+            if (b.kind == LPA_BIN_GLOBAL) gmax = std::max<uint64_t>(gmax, b.maxdeg);
+            q = r;
+        }
+        if (!csr) {
+            e->order_c = order;
+            // reorder the stream on the device
+            std::vector<uint64_t> noffs(cnt + 1, 0);
+            for (uint64_t q = 0; q < cnt; q++)
+                noffs[q + 1] = noffs[q] + (e->h_offs[order[q] + 1] - e->h_offs[order[q]]);
+            const uint64_t n64 = (e->nbits + 63) / 64 + 8;
+            e->words.alloc(n64);
+            HIPCHK(hipMemsetAsync(e->words.p, 0, n64 * 8, e->stream));
+            e->offs.upload(noffs.data(), cnt + 1, e->stream);
+            e->vid.alloc(std::max<uint64_t>(cnt, 1));
+            DevBuf<uint32_t> perm;
+            perm.upload(order.data(), cnt, e->stream);
+            if (cnt)
+                hipLaunchKernelGGL(k_relayout, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, e->stream,
+                                   e->words0.p, e->offs0.p, e->vid0.p, perm.p, e->offs.p, e->words.p, e->vid.p,
+                                   cnt);
+            HIPCHK(hipGetLastError());
+            e->sync();
+        } else {
+            e->order_s = order;
+            std::vector<uint64_t> row(cnt + 1, 0);
+            std::vector<uint32_t> nb;
+            std::vector<uint32_t> sv(cnt);
+            nb.reserve(e->h_snbr.size());
+            for (uint64_t q = 0; q < cnt; q++) {
+                const uint32_t i = order[q];
+                nb.insert(nb.end(), e->h_snbr.begin() + e->h_soffs[i], e->h_snbr.begin() + e->h_soffs[i + 1]);
+                row[q + 1] = nb.size();
+                sv[q] = e->h_vertices[e->n_c + i];
+            }
+            e->srow.upload(row.data(), cnt + 1, e->stream);
+            e->snbr.upload(nb.data(), nb.size(), e->stream);
+            e->svid.upload(sv.data(), cnt, e->stream);
+            e->sync();
+        }
+    }
+    if (gmax) {
+        e->gcap = 1ull << ceil_log2(2 * gmax);
+        e->gtab.alloc(e->gcap * e->gblocks);
+    } else {
+        e->gtab.release();
+        e->gcap = 0;
+    }
+    e->have_csr = false;
+    e->scheduled = true;
+    (void)p;
+}
+
+// LDS left for tables after the kernels' static shared memory
+static uint64_t lds_budget(const lpa_engine *e) { return e->lds_max - 8192; }
+// largest degree a WAVE (4 tables per workgroup) or BLOCK table can hold at load <= 1/2
+static uint32_t wave_cap(const lpa_engine *e) {
+    uint32_t l = 0;
+    while ((4ull * 8ull << (l + 1)) <= lds_budget(e)) l++;
+    return std::min<uint32_t>(1u << (l - 1), 1024);
+}
+static uint32_t block_cap(const lpa_engine *e) {
+    uint32_t l = 0;
+    while ((8ull << (l + 1)) <= lds_budget(e)) l++;
+    return 1u << (l - 1);
+}
+
+static uint8_t legal_kind(const lpa_engine *e, int want, uint32_t d, const lpa_sched_params &p) {
+    if (d == 0) return LPA_BIN_SKIP;
+    if (e->cms_thresh && d > e->cms_thresh) return LPA_BIN_CMS;
+    // promote to the first kind able to hold the degree
+    if (want <= LPA_BIN_LANE && d <= 32) return LPA_BIN_LANE;
+    if (want <= LPA_BIN_WAVE && d <= std::min<uint32_t>(p.wave_max ? p.wave_max : 1024, wave_cap(e))) return LPA_BIN_WAVE;
+    if (want <= LPA_BIN_BLOCK && d <= block_cap(e)) return LPA_BIN_BLOCK;
+    return LPA_BIN_GLOBAL;
+}
+
+int lpa_schedule_degree(lpa_engine *e, const lpa_sched_params *p) {
+    return guard([&] {
+        if (!e || !p) throw Fail(LPA_ERR_ARG, "null argument");
+        if (!e->have_graph) throw Fail(LPA_ERR_STATE, "no graph uploaded");
+        if (p->lane_max > 32) throw Fail(LPA_ERR_RANGE, "lane_max must be <= 32");
+        HIPCHK(hipSetDevice(e->device));
+        lpa_sched_params q = *p;
+        if (q.wave_max == 0) q.wave_max = 256;
+        if (q.block_max == 0) q.block_max = block_cap(e);
+        auto kind = [&](uint32_t d) -> uint8_t {
+            int want = d <= q.lane_max ? LPA_BIN_LANE : d <= q.wave_max ? LPA_BIN_WAVE
+                     : d <= q.block_max ? LPA_BIN_BLOCK : LPA_BIN_GLOBAL;
+            return legal_kind(e, want, d, q);
+        };
+        std::vector<uint8_t> kc(e->n_c), ks(e->n_s);
+        for (uint64_t i = 0; i < e->n_c; i++) kc[i] = kind(e->deg_c[i]);
+        for (uint64_t j = 0; j < e->n_s; j++) ks[j] = kind((uint32_t)(e->h_soffs[j + 1] - e->h_soffs[j]));
+        build_layout(e, kc, ks, q);
+    });
+}
+
+int lpa_schedule_bins(lpa_engine *e, const uint8_t *bins, uint32_t relayout) {
+    return guard([&] {
+        if (!e || !bins) throw Fail(LPA_ERR_ARG, "null argument");
+        if (!e->have_graph) throw Fail(LPA_ERR_STATE, "no graph uploaded");
+        HIPCHK(hipSetDevice(e->device));
+        lpa_sched_params q{32, 1024, 0, relayout};
+        std::vector<uint8_t> kc(e->n_c), ks(e->n_s);
+        for (uint64_t i = 0; i < e->n_c; i++) {
+            const int want = bins[e->h_vertices[i]];
+            if (want >= LPA_NBIN_KINDS) throw Fail(LPA_ERR_ARG, "bin kind out of range");
+            kc[i] = legal_kind(e, want, e->deg_c[i], q);
+        }
+        for (uint64_t j = 0; j < e->n_s; j++) {
+            const int want = bins[e->h_vertices[e->n_c + j]];
+            if (want >= LPA_NBIN_KINDS) throw Fail(LPA_ERR_ARG, "bin kind out of range");
+            ks[j] = legal_kind(e, want, (uint32_t)(e->h_soffs[j + 1] - e->h_soffs[j]), q);
+        }
+        build_layout(e, kc, ks, q);
+    });
+}
+
+int lpa_schedule_stats(lpa_engine *e, uint64_t *stats) {
+    return guard([&] {
+        if (!e || !stats) throw Fail(LPA_ERR_ARG, "null argument");
+        if (!e->scheduled) throw Fail(LPA_ERR_STATE, "not scheduled");
+        std::memset(stats, 0, 4 * LPA_NBIN_KINDS * sizeof(uint64_t));
+        for (const Bin &b : e->bins) {
+            uint64_t *s = stats + 4 * b.kind;
+            s[0] += b.end - b.begin;
+            for (uint32_t q = b.begin; q < b.end; q++) {
+                if (b.csr) {
+                    const uint32_t j = e->order_s[q];
+                    s[1] += e->h_soffs[j + 1] - e->h_soffs[j];
+                    s[3] += e->h_soffs[j + 1] - e->h_soffs[j];
+                } else {
+                    const uint32_t i = e->order_c[q];
+                    s[1] += e->deg_c[i];
+                    s[2] += e->h_offs[i + 1] - e->h_offs[i];
+                }
+            }
+        }
+    });
+}
+
+int lpa_schedule_counts(lpa_engine *e, uint64_t *counts) {
+    return guard([&] {
+        if (!e || !counts) throw Fail(LPA_ERR_ARG, "null argument");
+        if (!e->scheduled) throw Fail(LPA_ERR_STATE, "not scheduled");
+        std::memcpy(counts, e->bin_counts, sizeof e->bin_counts);
+    });
+}
+
+int lpa_labels_init(lpa_engine *e, const uint32_t *labels) {
+    return guard([&] {
+        if (!e) throw Fail(LPA_ERR_ARG, "null argument");
+        if (!e->have_graph) throw Fail(LPA_ERR_STATE, "no graph uploaded");
+        HIPCHK(hipSetDevice(e->device));
+        const uint64_t tot = e->n * e->K;
+        std::vector<uint32_t> h(tot);
+        if (labels) {
+            for (uint64_t i = 0; i < tot; i++) {
+                uint32_t x = labels[i];
+                if (x != UNL && (x & SEED_BIT)) throw Fail(LPA_ERR_RANGE, "labels must be < 2^31 or 0xFFFFFFFF");
+                h[i] = (e->semi && x != UNL) ? (x | SEED_BIT) : x;
+            }
+        } else {
+            for (uint64_t v = 0; v < e->n; v++)
+                for (uint32_t k = 0; k < e->K; k++) h[v * e->K + k] = (uint32_t)v | (e->semi ? SEED_BIT : 0u);
+        }
+        e->L[0].upload(h.data(), tot, e->stream);
+        e->L[1].upload(h.data(), tot, e->stream);
+        e->cur = 0;
+        e->sync();
+        e->have_labels = true;
+    });
+}
+
+int lpa_labels_get(lpa_engine *e, uint32_t *labels) {
+    return guard([&] {
+        if (!e || !labels) throw Fail(LPA_ERR_ARG, "null argument");
+        if (!e->have_labels) throw Fail(LPA_ERR_STATE, "no labels");
+        HIPCHK(hipSetDevice(e->device));
+        const uint64_t tot = e->n * e->K;
+        HIPCHK(hipMemcpyAsync(labels, e->L[e->cur].p, tot * 4, hipMemcpyDeviceToHost, e->stream));
+        e->sync();
+        for (uint64_t i = 0; i < tot; i++)
+            if (labels[i] != UNL) labels[i] &= ~SEED_BIT;
+    });
+}
+
+static void require_scheduled(lpa_engine *e) {
+    if (!e) throw Fail(LPA_ERR_ARG, "null argument");
+    if (!e->have_graph) throw Fail(LPA_ERR_STATE, "no graph uploaded");
+    if (!e->scheduled) throw Fail(LPA_ERR_STATE, "no schedule (call lpa_schedule_degree)");
+    HIPCHK(hipSetDevice(e->device));
+}
+
+static void require_ready(lpa_engine *e) {
+    require_scheduled(e);
+    if (!e->have_labels) throw Fail(LPA_ERR_STATE, "labels not initialised");
+}
+
+static void finish_step(lpa_engine *e, uint64_t *changed) {
+    HIPCHK(hipMemcpyAsync(e->h_counter, e->counter.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, e->stream));
+    e->sync();
+    e->cur ^= 1;
+    if (changed) *changed = *e->h_counter;
+}
+
+int lpa_nova_iterate(lpa_engine *e, uint64_t *changed) {
+    return guard([&] {
+        require_ready(e);
+        e->step(false, nullptr);
+        finish_step(e, changed);
+    });
+}
+
+static void cglp_decode(lpa_engine *e) {
+    // rows in device-layout order
+    std::vector<uint64_t> row(e->n_c + 1, 0);
+    for (uint64_t q = 0; q < e->n_c; q++) row[q + 1] = row[q] + e->deg_c[e->order_c[q]];
+    e->crow.upload(row.data(), e->n_c + 1, e->stream);
+    e->ccol.alloc(std::max<uint64_t>(row[e->n_c], 1));
+    for (const Bin &b : e->bins) {
+        if (b.csr || b.kind == LPA_BIN_SKIP || b.end == b.begin) continue;
+        const uint32_t count = b.end - b.begin;
+        if (b.maxdeg <= 256)
+            hipLaunchKernelGGL(k_decode_csr, dim3((count + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, e->stream,
+                               e->words.p, e->offs.p, e->vid.p, e->crow.p, e->ccol.p, b.begin, b.end);
+        else
+            hipLaunchKernelGGL(k_decode_csr_group, dim3(std::min<uint32_t>(count, 4096)), dim3(BLOCK), 0,
+                               e->stream, e->words.p, e->offs.p, e->vid.p, e->crow.p, e->ccol.p, b.begin, b.end);
+    }
+    HIPCHK(hipGetLastError());
+    e->have_csr = true;
+}
+
+int lpa_cglp_decode(lpa_engine *e) {
+    return guard([&] {
+        require_scheduled(e);
+        cglp_decode(e);
+        e->sync();
+    });
+}
+
+// The two-kernel step: decode the compressed part to CSR, then run the LP
+// kernels over the CSR (small vertices read their own CSR as in NOVA).
+static void cglp_step(lpa_engine *e) {
+    if (!e->have_csr) cglp_decode(e);
+    else {
+        for (const Bin &b : e->bins) {
+            if (b.csr || b.kind == LPA_BIN_SKIP || b.end == b.begin) continue;
+            const uint32_t count = b.end - b.begin;
+            if (b.maxdeg <= 256)
+                hipLaunchKernelGGL(k_decode_csr, dim3((count + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, e->stream,
+                                   e->words.p, e->offs.p, e->vid.p, e->crow.p, e->ccol.p, b.begin, b.end);
+            else
+                hipLaunchKernelGGL(k_decode_csr_group, dim3(std::min<uint32_t>(count, 4096)), dim3(BLOCK), 0,
+                                   e->stream, e->words.p, e->offs.p, e->vid.p, e->crow.p, e->ccol.p, b.begin,
+                                   b.end);
+        }
+    }
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, e->stream, e->counter.p, (uint64_t)1, 0ull);
+    StepArgs a = e->base_args(false);
+    StepArgs ac = e->base_args(true);
+    for (const Bin &b : e->bins) {
+        if (b.kind == LPA_BIN_SKIP) continue;
+        if (b.csr) e->launch_bin(b, a, true);
+        else e->launch_bin(b, ac, true);
+    }
+    HIPCHK(hipGetLastError());
+}
+
+int lpa_cglp_iterate(lpa_engine *e, uint64_t *changed) {
+    return guard([&] {
+        require_ready(e);
+        cglp_step(e);
+        finish_step(e, changed);
+    });
+}
+
+int lpa_cglp_csr_get(lpa_engine *e, uint64_t *row, uint32_t *col, uint64_t *m) {
+    return guard([&] {
+        require_scheduled(e);
+        if (!row || !m) throw Fail(LPA_ERR_ARG, "null argument");
+        if (!e->have_csr) cglp_decode(e);
+        e->sync();
+        std::vector<uint64_t> crow(e->n_c + 1);
+        std::vector<uint32_t> vid(e->n_c);
+        HIPCHK(hipMemcpy(crow.data(), e->crow.p, crow.size() * 8, hipMemcpyDeviceToHost));
+        if (e->n_c) HIPCHK(hipMemcpy(vid.data(), e->vid.p, e->n_c * 4, hipMemcpyDeviceToHost));
+        std::vector<uint32_t> ccol(crow[e->n_c]);
+        if (!ccol.empty()) HIPCHK(hipMemcpy(ccol.data(), e->ccol.p, ccol.size() * 4, hipMemcpyDeviceToHost));
+        // by vertex id: compressed rows from the device, small rows from the host CSR
+        std::vector<uint64_t> deg(e->n, 0), src(e->n, 0);
+        std::vector<int8_t> kind(e->n, -1);
+        for (uint64_t q = 0; q < e->n_c; q++) {
+            deg[vid[q]] = crow[q + 1] - crow[q];
+            src[vid[q]] = q;
+            kind[vid[q]] = 0;
+        }
+        for (uint64_t j = 0; j < e->n_s; j++) {
+            uint32_t v = e->h_vertices[e->n_c + j];
+            deg[v] = e->h_soffs[j + 1] - e->h_soffs[j];
+            src[v] = j;
+            kind[v] = 1;
+        }
+        row[0] = 0;
+        for (uint64_t v = 0; v < e->n; v++) row[v + 1] = row[v] + deg[v];
+        *m = row[e->n];
+        if (!col) return;
+        for (uint64_t v = 0; v < e->n; v++) {
+            const uint32_t *s = kind[v] == 0 ? &ccol[crow[src[v]]] : &e->h_snbr[e->h_soffs[src[v]]];
+            std::memcpy(col + row[v], s, deg[v] * 4);
+        }
+    });
+}
+
+int lpa_time_iterations(lpa_engine *e, int method, int iters, int use_graph, float *ms, uint64_t *changed) {
+    return guard([&] {
+        require_ready(e);
+        if (iters <= 0 || !ms) throw Fail(LPA_ERR_ARG, "bad arguments");
+        hipEvent_t ev[2];
+        hipEvent_t kev[2 * LPA_NBIN_KINDS];
+        for (auto &x : ev) HIPCHK(hipEventCreate(&x));
+        for (auto &x : kev) HIPCHK(hipEventCreate(&x));
+        std::vector<double> kind_sum(LPA_NBIN_KINDS, 0.0);
+        try {
+            if (method == 1 && !e->have_csr) {
+                cglp_decode(e);
+                e->sync();
+            }
+            for (int it = 0; it < iters; it++) {
+                HIPCHK(hipEventRecord(ev[0], e->stream));
+                if (method == 0) e->step(false, kev);
+                else cglp_step(e);
+                HIPCHK(hipEventRecord(ev[1], e->stream));
+                HIPCHK(hipMemcpyAsync(e->h_counter, e->counter.p, 8, hipMemcpyDeviceToHost, e->stream));
+                e->sync();
+                HIPCHK(hipEventElapsedTime(&ms[it], ev[0], ev[1]));
+                if (changed) changed[it] = *e->h_counter;
+                e->cur ^= 1;
+                if (method == 0) {
+                    bool seen[LPA_NBIN_KINDS] = {false};
+                    for (const Bin &b : e->bins)
+                        if (b.kind != LPA_BIN_SKIP && !seen[b.kind] && b.end > b.begin) {
+                            seen[b.kind] = true;
+                            float t = 0;
+                            if (hipEventElapsedTime(&t, kev[2 * b.kind], kev[2 * b.kind + 1]) == hipSuccess)
+                                kind_sum[b.kind] += t;
+                        }
+                }
+            }
+        } catch (...) {
+            for (auto &x : ev) (void)hipEventDestroy(x);
+            for (auto &x : kev) (void)hipEventDestroy(x);
+            throw;
+        }
+        for (int k = 0; k < LPA_NBIN_KINDS; k++) e->kind_ms[k] = (float)(kind_sum[k] / iters);
+        for (auto &x : ev) HIPCHK(hipEventDestroy(x));
+        for (auto &x : kev) HIPCHK(hipEventDestroy(x));
+        (void)use_graph;
+    });
+}
+
+int lpa_last_kernel_times(lpa_engine *e, float *ms_per_kind) {
+    return guard([&] {
+        if (!e || !ms_per_kind) throw Fail(LPA_ERR_ARG, "null argument");
+        std::memcpy(ms_per_kind, e->kind_ms, sizeof e->kind_ms);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,439 @@
+// lp_kernels.hpp -- gfx950 kernels of one synchronous label-propagation step.
+//
+// The reference computes each vertex's label with three kernel shapes picked
+// by degree (baselines/novaplus/kernel.cuh:14-191): a block per vertex of
+// degree >= 128 with a count-min sketch (include/decoder.cuh:103-419), a warp
+// per medium vertex with an LDS hash (decoder.cuh:618-800), and warp kernels
+// over an uncompressed CSR for small vertices.  It updates labels in place,
+// prefers larger labels on ties and drops colliding labels.
+//
+// Here every shape computes the exact LP-SC result (double-buffered labels,
+// most frequent label, ties to the smallest) and shapes differ only in
+// speed:
+//   k_lane   one lane per vertex (deg <= 32): serial decode into an LDS
+//            list, labels gathered into registers, bitonic sorting network,
+//            longest run.
+//   k_group  G lanes per vertex (G = 64: one wave, G = 256: one workgroup):
+//            interval and residual segments (256-bit units) decoded one per
+//            lane, each decoded neighbour's label counted in an open-address
+//            hash table (LDS, or HBM for very large degree) with 64-bit
+//            atomics on (label << 32 | count) words; or, for vertices above
+//            the CMS threshold, a count-min sketch in LDS filled in one
+//            decode pass and queried in a second.
+// A lane keeps max((count << 32) | ~label) over every count it produced, so
+// the group maximum names the most frequent label and, among equal counts,
+// the smallest one, without a pass over the table.
+//
+// Labels: layout L[v*K + k]; bit 31 marks a clamped seed (semi-supervised),
+// 0xFFFFFFFF an unlabelled entry.  Bins are contiguous ranges of the
+// device layout (records reordered by bin, then degree).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cgr_decode.hpp"
+
+namespace lpa {
+
+constexpr uint32_t UNL = 0xFFFFFFFFu;
+constexpr uint32_t SEED_BIT = 0x80000000u;
+constexpr uint64_t EMPTY_SLOT = 0xFFFFFFFF00000000ull;
+constexpr int BLOCK = 256;
+
+struct StepArgs {
+    // compressed part (device layout)
+    const uint64_t *__restrict__ words;
+    const uint64_t *__restrict__ offs;     // start bit offset per position
+    const uint32_t *__restrict__ vid;      // vertex id per position
+    // CSR part (small vertices, or the CGLP decoded CSR)
+    const uint64_t *__restrict__ row;      // per position: [row[p], row[p+1])
+    const uint32_t *__restrict__ nbr;
+    const uint32_t *__restrict__ cvid;     // vertex id per CSR position
+    // labels
+    const uint32_t *__restrict__ lin;
+    uint32_t *__restrict__ lout;
+    uint32_t K;
+    // CMS (row hashes a_r, b_r)
+    uint32_t cms_depth, cms_logw;
+    uint64_t cms_ab[16];
+    // scratch hash tables in HBM: gcap entries per workgroup
+    uint64_t *__restrict__ gtab;
+    uint64_t gcap;
+    unsigned long long *__restrict__ changed;
+    uint32_t begin, end;                   // position range of this bin
+};
+
+__device__ __forceinline__ uint32_t clean_label(uint32_t raw) {
+    return raw == UNL ? UNL : (raw & ~SEED_BIT);
+}
+
+__device__ __forceinline__ uint32_t pick(uint64_t best, uint32_t old) {
+    // best == 0: no labelled neighbour -> keep; seeds stay clamped
+    if ((old != UNL && (old & SEED_BIT)) || best == 0) return old;
+    return ~(uint32_t)best;
+}
+
+__device__ __forceinline__ void count_changes(bool ch, unsigned long long *counter) {
+    uint64_t b = __ballot(ch);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(counter, (unsigned long long)__popcll(b));
+}
+
+// ------------------------------------------------------------ lane kernel
+
+template <int N>
+__device__ __forceinline__ void bitonic_sort(uint32_t (&x)[N]) {
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                const int l = i ^ j;
+                if (l > i) {
+                    uint32_t a = x[i], b = x[l];
+                    uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+                    if ((i & k) == 0) { x[i] = lo; x[l] = hi; }
+                    else { x[i] = hi; x[l] = lo; }
+                }
+            }
+}
+
+// longest run of a sorted array; UNL entries (sorted last) never win
+template <int N>
+__device__ __forceinline__ uint64_t longest_run(const uint32_t (&x)[N]) {
+    uint32_t best = 0, bestc = 0, run = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        run = (i > 0 && x[i] == x[i - 1]) ? run + 1 : 1;
+        bool win = x[i] != UNL && run > bestc;
+        bestc = win ? run : bestc;
+        best = win ? x[i] : best;
+    }
+    return bestc ? ((uint64_t)bestc << 32) | (uint32_t)~best : 0;
+}
+
+// CSR=false: neighbours decoded from the CGR record; CSR=true: read from row/nbr.
+template <int N, bool CSR>
+__global__ void __launch_bounds__(BLOCK) k_lane(StepArgs a) {
+    __shared__ uint32_t list[CSR ? 1 : N][BLOCK];
+    const uint32_t p = a.begin + blockIdx.x * BLOCK + threadIdx.x;
+    const bool active = p < a.end;
+    uint32_t v = 0, d = 0;
+    uint64_t base = 0;
+    if (active) {
+        if (CSR) {
+            v = a.cvid[p];
+            base = a.row[p];
+            d = (uint32_t)(a.row[p + 1] - base);
+        } else {
+            v = a.vid[p];
+            decode_record(a.words, a.offs[p], v, [&](uint32_t u) {
+                if (d < N) list[d][threadIdx.x] = u;
+                d++;
+            });
+        }
+    }
+    for (uint32_t k = 0; k < a.K; k++) {
+        uint32_t x[N];
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            uint32_t u = 0;
+            if ((uint32_t)j < d) u = CSR ? a.nbr[base + j] : list[CSR ? 0 : j][threadIdx.x];
+            x[j] = (uint32_t)j < d ? clean_label(a.lin[(uint64_t)u * a.K + k]) : UNL;
+        }
+        bitonic_sort<N>(x);
+        uint64_t best = longest_run<N>(x);
+        bool ch = false;
+        if (active) {
+            uint64_t idx = (uint64_t)v * a.K + k;
+            uint32_t old = a.lin[idx];
+            uint32_t nl = pick(best, old);
+            a.lout[idx] = nl;
+            ch = nl != old;
+        }
+        count_changes(ch, a.changed);
+    }
+}
+
+// ------------------------------------------------------------ group kernels
+
+enum { MODE_LDS = 0, MODE_GLOBAL = 1, MODE_CMS = 2 };
+
+template <int G>
+__device__ __forceinline__ void group_sync() {
+    if (G == BLOCK) {
+        __syncthreads();
+    } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        uint64_t y = __shfl_xor(x, o, 64);
+        x = y > x ? y : x;
+    }
+    return x;
+}
+
+// Add `cnt` occurrences of lab to the table (2^logT slots); returns the
+// label's count after the addition.
+__device__ __forceinline__ uint32_t tab_add(uint64_t *tab, uint32_t logT, uint32_t lab, uint32_t cnt) {
+    const uint32_t mask = (1u << logT) - 1;
+    uint32_t h = logT ? (lab * 2654435769u) >> (32 - logT) : 0;
+    for (;;) {
+        uint64_t e = __hip_atomic_load(&tab[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t key = (uint32_t)(e >> 32);
+        if (key == lab) return (uint32_t)atomicAdd((unsigned long long *)&tab[h], (unsigned long long)cnt) + cnt;
+        if (key == UNL) {
+            uint64_t old = atomicCAS((unsigned long long *)&tab[h], EMPTY_SLOT, ((uint64_t)lab << 32) | cnt);
+            if (old == EMPTY_SLOT) return cnt;
+            if ((uint32_t)(old >> 32) == lab)
+                return (uint32_t)atomicAdd((unsigned long long *)&tab[h], (unsigned long long)cnt) + cnt;
+        }
+        h = (h + 1) & mask;
+    }
+}
+
+// Wave-level pre-aggregation: the active lanes holding the same label as the
+// first active lane are folded into one update by the lowest of them.
+// Returns the number of occurrences this lane must apply (0: folded away).
+__device__ __forceinline__ uint32_t fold_first_label(uint32_t lab) {
+    const uint32_t first = __builtin_amdgcn_readfirstlane(lab);
+    const uint64_t m = __ballot(lab == first);
+    if (lab != first) return 1;
+    const uint32_t lane = __lane_id();
+    return lane == (uint32_t)__builtin_ctzll(m) ? (uint32_t)__popcll(m) : 0;
+}
+
+__device__ __forceinline__ uint32_t cms_col(const StepArgs &a, uint32_t r, uint32_t lab) {
+    uint64_t h = a.cms_ab[2 * r] * (uint64_t)lab + a.cms_ab[2 * r + 1];
+    return a.cms_logw ? (uint32_t)(h >> (64 - a.cms_logw)) : 0;
+}
+
+// Intervals at least this long are expanded by the whole group.
+constexpr uint32_t LONG_RUN = 16;
+
+template <int G> struct GroupShared {
+    static constexpr int QCAP = G == BLOCK ? 512 : 64;
+    uint64_t res_start;
+    uint32_t qn;
+    uint32_t qleft[QCAP];
+    uint32_t qlen[QCAP];
+};
+
+// Visit every neighbour of the vertex at position p with the G lanes of a
+// group: CSR rows are strided over lanes; CGR records are split into their
+// 256-bit segments, one per lane (interval block first, then the residual
+// block, which starts where the last interval segment ends).  Intervals of
+// LONG_RUN or more ids are queued and expanded by all lanes together, so a
+// hub whose neighbours form one long run is not walked by a single lane.
+template <int G, bool CSR, class F>
+__device__ __forceinline__ void group_neighbours(const StepArgs &a, uint32_t p, uint32_t v, uint32_t gl,
+                                                 GroupShared<G> *sh, F &&visit) {
+    if (CSR) {
+        const uint64_t b = a.row[p], e = a.row[p + 1];
+        for (uint64_t i = b + gl; i < e; i += G) visit(a.nbr[i]);
+        return;
+    }
+    if (gl == 0) sh->qn = 0;
+    group_sync<G>();
+    uint64_t base;
+    const uint32_t ni = block_header(a.words, a.offs[p], &base);
+    uint64_t my_end = 0;
+    for (uint32_t j = gl; j < ni; j += G) {
+        uint64_t e = decode_interval_segment(a.words, base + (uint64_t)SEG_BITS * j, v,
+                                             [&](uint32_t l, uint32_t len) {
+                                                 if (len >= LONG_RUN) {
+                                                     uint32_t q = atomicAdd(&sh->qn, 1u);
+                                                     if (q < GroupShared<G>::QCAP) {
+                                                         sh->qleft[q] = l;
+                                                         sh->qlen[q] = len;
+                                                         return;
+                                                     }
+                                                 }
+                                                 for (uint32_t x = 0; x < len; x++) visit(l + x);
+                                             });
+        if (j == ni - 1) my_end = e;
+    }
+    if (G == 64) {
+        if (gl == 0) sh->res_start = 0;   // unused for waves; keeps the slot defined
+    }
+    uint64_t rs;
+    if (G == 64) {
+        rs = __shfl(my_end, (int)((ni - 1) & 63), 64);
+    } else {
+        if (gl == ((ni - 1) & (G - 1))) sh->res_start = my_end;
+    }
+    group_sync<G>();
+    if (G != 64) rs = sh->res_start;
+    const uint32_t nq = sh->qn < (uint32_t)GroupShared<G>::QCAP ? sh->qn : (uint32_t)GroupShared<G>::QCAP;
+    for (uint32_t q = 0; q < nq; q++) {
+        const uint32_t l = sh->qleft[q], len = sh->qlen[q];
+        for (uint32_t x = gl; x < len; x += G) visit(l + x);
+    }
+    const uint32_t nr = block_header(a.words, rs, &base);
+    for (uint32_t j = gl; j < nr; j += G)
+        decode_residual_segment(a.words, base + (uint64_t)SEG_BITS * j, v, visit);
+}
+
+// One group of G lanes per vertex.  LDS holds, per group, 2^logT table slots
+// (MODE_LDS), or the CMS sketch (MODE_CMS, per workgroup).
+template <int G, int MODE, bool CSR>
+__global__ void __launch_bounds__(BLOCK) k_group(StepArgs a, uint32_t logT) {
+    constexpr int NG = BLOCK / G;
+    extern __shared__ uint64_t lds[];
+    __shared__ uint64_t red[BLOCK / 64];
+    __shared__ GroupShared<G> gsh[NG];
+    const uint32_t gid = threadIdx.x / G, gl = threadIdx.x % G;
+    const uint32_t T = 1u << logT;
+    uint64_t *tab;
+    uint32_t *sketch = (uint32_t *)lds;
+    if (MODE == MODE_GLOBAL) tab = a.gtab + (uint64_t)blockIdx.x * a.gcap;
+    else tab = lds + (uint64_t)gid * T;
+    GroupShared<G> *sh = &gsh[gid];
+    const uint32_t stride = gridDim.x * NG;
+    for (uint32_t p = a.begin + blockIdx.x * NG + gid; ; p += stride) {
+        // every group of a workgroup runs the same trip count when G == BLOCK
+        if (p >= a.end) break;
+        const uint32_t v = CSR ? a.cvid[p] : a.vid[p];
+        for (uint32_t k = 0; k < a.K; k++) {
+            uint64_t best = 0;
+            if (MODE == MODE_CMS) {
+                const uint32_t cells = a.cms_depth << a.cms_logw;
+                for (uint32_t i = gl; i < cells; i += G) sketch[i] = 0;
+                group_sync<G>();
+                group_neighbours<G, CSR>(a, p, v, gl, sh, [&](uint32_t u) {
+                    uint32_t lab = clean_label(a.lin[(uint64_t)u * a.K + k]);
+                    if (lab == UNL) return;
+                    const uint32_t cnt = fold_first_label(lab);
+                    if (cnt == 0) return;
+                    for (uint32_t r = 0; r < a.cms_depth; r++)
+                        atomicAdd(&sketch[(r << a.cms_logw) + cms_col(a, r, lab)], cnt);
+                });
+                group_sync<G>();
+                group_neighbours<G, CSR>(a, p, v, gl, sh, [&](uint32_t u) {
+                    uint32_t lab = clean_label(a.lin[(uint64_t)u * a.K + k]);
+                    if (lab == UNL) return;
+                    uint32_t est = UNL;
+                    for (uint32_t r = 0; r < a.cms_depth; r++) {
+                        uint32_t c = sketch[(r << a.cms_logw) + cms_col(a, r, lab)];
+                        est = c < est ? c : est;
+                    }
+                    uint64_t key = ((uint64_t)est << 32) | (uint32_t)~lab;
+                    best = key > best ? key : best;
+                });
+            } else {
+                for (uint32_t i = gl; i < T; i += G) tab[i] = EMPTY_SLOT;
+                if (MODE == MODE_GLOBAL) __threadfence_block();
+                group_sync<G>();
+                group_neighbours<G, CSR>(a, p, v, gl, sh, [&](uint32_t u) {
+                    uint32_t lab = clean_label(a.lin[(uint64_t)u * a.K + k]);
+                    if (lab == UNL) return;
+                    const uint32_t cnt = fold_first_label(lab);
+                    if (cnt == 0) return;
+                    uint32_t c = tab_add(tab, logT, lab, cnt);
+                    uint64_t key = ((uint64_t)c << 32) | (uint32_t)~lab;
+                    best = key > best ? key : best;
+                });
+            }
+            best = wave_max_u64(best);
+            if (G == BLOCK) {
+                if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = best;
+                __syncthreads();
+#pragma unroll
+                for (int w = 0; w < BLOCK / 64; w++) best = red[w] > best ? red[w] : best;
+            }
+            if (gl == 0) {
+                uint64_t idx = (uint64_t)v * a.K + k;
+                uint32_t old = a.lin[idx];
+                uint32_t nl = pick(best, old);
+                a.lout[idx] = nl;
+                if (nl != old) atomicAdd(a.changed, 1ull);
+            }
+            group_sync<G>();   // table / sketch / red / queue reused next
+        }
+    }
+}
+
+// --------------------------------------------------------- CGLP kernel 1
+
+// Decode the records of positions [begin,end) into CSR rows crow[p]..crow[p+1]
+// (rows follow the device layout; one lane per record; the two-kernel
+// method's decode step).
+__global__ void __launch_bounds__(BLOCK) k_decode_csr(const uint64_t *__restrict__ words,
+                                                      const uint64_t *__restrict__ offs,
+                                                      const uint32_t *__restrict__ vid,
+                                                      const uint64_t *__restrict__ crow,
+                                                      uint32_t *__restrict__ ccol, uint32_t begin, uint32_t end) {
+    const uint32_t p = begin + blockIdx.x * BLOCK + threadIdx.x;
+    if (p >= end) return;
+    const uint32_t v = vid[p];
+    uint64_t o = crow[p];
+    decode_record(words, offs[p], v, [&](uint32_t u) { ccol[o++] = u; });
+}
+
+// Same for long records: a workgroup per record, one lane per segment;
+// output slots are claimed with an LDS cursor, and intervals of LONG_RUN or
+// more ids are written by all lanes together.
+__global__ void __launch_bounds__(BLOCK) k_decode_csr_group(const uint64_t *__restrict__ words,
+                                                            const uint64_t *__restrict__ offs,
+                                                            const uint32_t *__restrict__ vid,
+                                                            const uint64_t *__restrict__ crow,
+                                                            uint32_t *__restrict__ ccol, uint32_t begin,
+                                                            uint32_t end) {
+    constexpr int QCAP = 512;
+    __shared__ uint64_t res_start;
+    __shared__ unsigned long long cursor;
+    __shared__ uint32_t qn;
+    __shared__ uint32_t qleft[QCAP], qlen[QCAP];
+    __shared__ uint64_t qout[QCAP];
+    for (uint32_t p = begin + blockIdx.x; p < end; p += gridDim.x) {
+        const uint32_t v = vid[p];
+        if (threadIdx.x == 0) {
+            cursor = crow[p];
+            qn = 0;
+        }
+        __syncthreads();
+        uint64_t base;
+        const uint32_t ni = block_header(words, offs[p], &base);
+        uint64_t my_end = 0;
+        for (uint32_t j = threadIdx.x; j < ni; j += BLOCK) {
+            uint64_t e = decode_interval_segment(words, base + (uint64_t)SEG_BITS * j, v,
+                                                 [&](uint32_t l, uint32_t len) {
+                                                     uint64_t o = atomicAdd(&cursor, (unsigned long long)len);
+                                                     if (len >= LONG_RUN) {
+                                                         uint32_t q = atomicAdd(&qn, 1u);
+                                                         if (q < QCAP) {
+                                                             qleft[q] = l;
+                                                             qlen[q] = len;
+                                                             qout[q] = o;
+                                                             return;
+                                                         }
+                                                     }
+                                                     for (uint32_t x = 0; x < len; x++) ccol[o + x] = l + x;
+                                                 });
+            if (j == ni - 1) my_end = e;
+        }
+        if (threadIdx.x == ((ni - 1) & (BLOCK - 1))) res_start = my_end;
+        __syncthreads();
+        const uint32_t nq = qn < (uint32_t)QCAP ? qn : (uint32_t)QCAP;
+        for (uint32_t q = 0; q < nq; q++)
+            for (uint32_t x = threadIdx.x; x < qlen[q]; x += BLOCK) ccol[qout[q] + x] = qleft[q] + x;
+        const uint32_t nr = block_header(words, res_start, &base);
+        for (uint32_t j = threadIdx.x; j < nr; j += BLOCK) {
+            const uint64_t q = base + (uint64_t)SEG_BITS * j;
+            BitReader r;
+            r.seek(words, q);
+            const uint32_t c = r.gamma();
+            uint64_t o = atomicAdd(&cursor, (unsigned long long)c);
+            decode_residual_segment(words, q, v, [&](uint32_t u) { ccol[o++] = u; });
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace lpa

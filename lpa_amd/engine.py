@@ -1,0 +1,200 @@
+"""ctypes binding of liblpa_hip.so (include/lpa_amd.h): the gfx950 engine.
+
+The product path: every call lands in the HIP library; there is no CPU
+fallback.  Loading fails loudly when the library is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .host import Cgr, LpaError
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_VP = C.c_void_p
+
+BIN_SKIP, BIN_LANE, BIN_WAVE, BIN_BLOCK, BIN_GLOBAL, BIN_CMS = range(6)
+BIN_NAMES = ["skip", "lane", "wave", "block", "global", "cms"]
+UNLABELLED = 0xFFFFFFFF
+
+EXPORTS = {
+    "lpa_last_error": (C.c_char_p, []),
+    "lpa_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "lpa_engine_create": (C.c_int, [C.c_int, C.POINTER(_VP)]),
+    "lpa_engine_destroy": (C.c_int, [_VP]),
+    "lpa_graph_upload": (C.c_int, [_VP, C.c_uint64, C.c_uint64, _VP, C.c_uint64, _VP, _VP, _VP, _VP]),
+    "lpa_graph_degrees": (C.c_int, [_VP, _VP]),
+    "lpa_graph_features": (C.c_int, [_VP, _VP]),
+    "lpa_lp_config": (C.c_int, [_VP, C.c_uint32, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64]),
+    "lpa_schedule_degree": (C.c_int, [_VP, _VP]),
+    "lpa_schedule_bins": (C.c_int, [_VP, _VP, C.c_uint32]),
+    "lpa_schedule_counts": (C.c_int, [_VP, _VP]),
+    "lpa_schedule_stats": (C.c_int, [_VP, _VP]),
+    "lpa_labels_init": (C.c_int, [_VP, _VP]),
+    "lpa_labels_get": (C.c_int, [_VP, _VP]),
+    "lpa_nova_iterate": (C.c_int, [_VP, C.POINTER(C.c_uint64)]),
+    "lpa_cglp_decode": (C.c_int, [_VP]),
+    "lpa_cglp_iterate": (C.c_int, [_VP, C.POINTER(C.c_uint64)]),
+    "lpa_cglp_csr_get": (C.c_int, [_VP, _VP, _VP, C.POINTER(C.c_uint64)]),
+    "lpa_time_iterations": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, _VP, _VP]),
+    "lpa_last_kernel_times": (C.c_int, [_VP, _VP]),
+}
+
+
+class SchedParams(C.Structure):
+    _fields_ = [("lane_max", C.c_uint32), ("wave_max", C.c_uint32),
+                ("block_max", C.c_uint32), ("relayout", C.c_uint32)]
+
+
+def library_path():
+    return os.path.join(_HERE, "liblpa_hip.so")
+
+
+def load():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = library_path()
+    if not os.path.exists(path):
+        raise LpaError(f"{path} is not built: the HIP engine is required (no CPU fallback)")
+    L = C.CDLL(path)
+    for name, (res, args) in EXPORTS.items():
+        f = getattr(L, name)
+        f.restype, f.argtypes = res, args
+    _LIB = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise LpaError(f"lpa engine error {rc}: {load().lpa_last_error().decode()}")
+
+
+def device_count():
+    c = C.c_int(0)
+    _check(load().lpa_device_count(C.byref(c)))
+    return c.value
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+class Engine:
+    """One engine per device: graph, schedule, label buffers, stream."""
+
+    def __init__(self, device=0):
+        self._L = load()
+        h = _VP()
+        _check(self._L.lpa_engine_create(device, C.byref(h)))
+        self._h = h
+        self.n = 0
+        self.K = 1
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.lpa_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    # graph -------------------------------------------------------------
+    def upload(self, cgr: Cgr):
+        self.upload_arrays(cgr.n, cgr.n_c, cgr.words, cgr.nwords, cgr.offsets, cgr.vertices,
+                           cgr.small_offsets, cgr.small_nbrs)
+
+    def upload_arrays(self, n, n_c, words32, nwords, offsets, vertices, small_offsets=None,
+                      small_nbrs=None):
+        words32 = np.ascontiguousarray(words32, np.uint32)
+        offsets = np.ascontiguousarray(offsets, np.uint64)
+        vertices = np.ascontiguousarray(vertices, np.uint32)
+        so = None if small_offsets is None else np.ascontiguousarray(small_offsets, np.uint64)
+        sn = None if small_nbrs is None else np.ascontiguousarray(small_nbrs, np.uint32)
+        if sn is not None and len(sn) == 0:
+            sn = np.zeros(1, np.uint32)
+        _check(self._L.lpa_graph_upload(self._h, n, n_c, _p(words32), nwords, _p(offsets),
+                                        _p(vertices), _p(so), _p(sn)))
+        self.n = n
+
+    def degrees(self):
+        d = np.zeros(self.n, np.uint32)
+        _check(self._L.lpa_graph_degrees(self._h, _p(d)))
+        return d
+
+    def features(self):
+        f = np.zeros((self.n, 8), np.float32)
+        _check(self._L.lpa_graph_features(self._h, _p(f)))
+        return f
+
+    # configuration -----------------------------------------------------
+    def config(self, K=1, semi=False, cms_thresh=0, cms_depth=4, cms_width=1024, cms_seed=0):
+        _check(self._L.lpa_lp_config(self._h, K, int(semi), cms_thresh, cms_depth, cms_width, cms_seed))
+        self.K = K
+
+    def schedule_degree(self, lane_max=32, wave_max=256, block_max=0):
+        p = SchedParams(lane_max, wave_max, block_max, 1)
+        _check(self._L.lpa_schedule_degree(self._h, C.byref(p)))
+
+    def schedule_bins(self, bins):
+        b = np.ascontiguousarray(bins, np.uint8)
+        _check(self._L.lpa_schedule_bins(self._h, _p(b), 1))
+
+    def counts(self):
+        c = np.zeros(6, np.uint64)
+        _check(self._L.lpa_schedule_counts(self._h, _p(c)))
+        return dict(zip(BIN_NAMES, c.tolist()))
+
+    def stats(self):
+        """per bin kind: vertices, arcs, stream bits, small-CSR arcs"""
+        st = np.zeros((6, 4), np.uint64)
+        _check(self._L.lpa_schedule_stats(self._h, _p(st)))
+        return {name: dict(zip(("vertices", "arcs", "bits", "csr_arcs"), map(int, st[i])))
+                for i, name in enumerate(BIN_NAMES)}
+
+    # labels --------------------------------------------------------------
+    def labels_init(self, labels=None):
+        lab = None if labels is None else np.ascontiguousarray(labels, np.uint32).reshape(-1)
+        if lab is not None and len(lab) != self.n * self.K:
+            raise LpaError("labels must have n*K entries")
+        _check(self._L.lpa_labels_init(self._h, _p(lab)))
+
+    def labels(self):
+        out = np.zeros(self.n * self.K, np.uint32)
+        _check(self._L.lpa_labels_get(self._h, _p(out)))
+        return out
+
+    # steps ---------------------------------------------------------------
+    def nova_iterate(self):
+        c = C.c_uint64(0)
+        _check(self._L.lpa_nova_iterate(self._h, C.byref(c)))
+        return c.value
+
+    def cglp_decode(self):
+        _check(self._L.lpa_cglp_decode(self._h))
+
+    def cglp_iterate(self):
+        c = C.c_uint64(0)
+        _check(self._L.lpa_cglp_iterate(self._h, C.byref(c)))
+        return c.value
+
+    def cglp_csr(self):
+        row = np.zeros(self.n + 1, np.uint64)
+        m = C.c_uint64(0)
+        _check(self._L.lpa_cglp_csr_get(self._h, _p(row), None, C.byref(m)))
+        col = np.zeros(max(m.value, 1), np.uint32)
+        _check(self._L.lpa_cglp_csr_get(self._h, _p(row), _p(col), C.byref(m)))
+        return row, col[:m.value]
+
+    def time_iterations(self, iters, method="nova", use_graph=False):
+        ms = np.zeros(iters, np.float32)
+        ch = np.zeros(iters, np.uint64)
+        _check(self._L.lpa_time_iterations(self._h, 0 if method == "nova" else 1, iters,
+                                           int(use_graph), _p(ms), _p(ch)))
+        return ms, ch
+
+    def kernel_times(self):
+        t = np.zeros(6, np.float32)
+        _check(self._L.lpa_last_kernel_times(self._h, _p(t)))
+        return dict(zip(BIN_NAMES, t.tolist()))

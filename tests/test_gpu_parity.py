@@ -1,0 +1,91 @@
+"""GPU parity: the gfx950 engine against the CPU oracle on the same inputs,
+bit for bit, every iteration, through the C ABI."""
+import os
+
+import numpy as np
+import pytest
+
+import lpa_amd as L
+from gpu_common import make_engine, oracle_view, run_and_compare
+from helpers import golden_cases, load_golden
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.mark.parametrize("case", golden_cases())
+def test_reference_file_sets(case):
+    g = load_golden(case)
+    cgr = L.Cgr.load(os.path.join(GOLDEN, case, "edges.txt"), g["params"]["small_thresh"])
+    run_and_compare(cgr, 5)
+
+
+@pytest.mark.parametrize("sched", [dict(lane_max=32, wave_max=256), dict(lane_max=0, wave_max=0, block_max=1),
+                                   dict(lane_max=0, wave_max=512), dict(lane_max=8, wave_max=32, block_max=64)])
+def test_each_kernel_family(sched):
+    g = L.gen_rmat(13, 16, seed=11)
+    run_and_compare(g.encode(0, 128), 4, **sched)
+
+
+def test_device_degrees_and_decode_to_csr():
+    g = L.gen_rmat(14, 16, seed=12)
+    cgr = g.encode(0, 128)
+    e = make_engine(cgr)
+    assert np.array_equal(e.degrees(), g.degrees().astype(np.uint32))
+    row, col = e.cglp_csr()
+    assert np.array_equal(row, g.row)
+    for v in range(0, g.n, 97):
+        assert np.array_equal(np.sort(col[row[v]:row[v + 1]]), g.col[g.row[v]:g.row[v + 1]])
+
+
+def test_lattice_config_full_size():
+    """1024x1024 lattice, p_drop 0.10, 0.5% shortcuts, seed 0; 10 iterations"""
+    g = L.gen_lattice(1024, 0.10, 0.005, seed=0)
+    run_and_compare(g.encode(0, 128), 10)
+
+
+def test_cglp_method_matches():
+    g = L.gen_rmat(13, 16, seed=13)
+    run_and_compare(g.encode(0, 128), 4, method="cglp")
+
+
+def test_cms_vertices():
+    g = L.gen_rmat(14, 16, seed=14)
+    run_and_compare(g.encode(0, 128), 4, cms_thresh=64, cms_depth=4, cms_width=256, cms_seed=7)
+
+
+def test_batched_and_semi_supervised():
+    g = L.gen_rmat(12, 16, seed=15)
+    cgr = g.encode(0, 128)
+    rng = np.random.default_rng(0)
+    K = 5
+    lab = np.full(g.n * K, 0xFFFFFFFF, np.uint32)
+    seed_mask = rng.random(g.n * K) < 0.05
+    lab[seed_mask] = rng.integers(0, 20, seed_mask.sum())
+    run_and_compare(cgr, 5, labels0=lab, K=K, semi=True)
+    run_and_compare(cgr, 3, K=3)
+
+
+def test_small_vertex_csr_part():
+    g = L.gen_copying(20000, 30.0, 2.1, 3000, 0.7, 200, seed=2)
+    run_and_compare(g.encode(32, 128), 4, cms_thresh=1000, cms_depth=4, cms_width=512, cms_seed=3)
+
+
+def test_global_hash_bin():
+    g = L.gen_rmat(15, 16, seed=16)
+    cgr = g.encode(0, 128)
+    bins = np.full(g.n, L.engine.BIN_GLOBAL, np.uint8)
+    run_and_compare(cgr, 3, schedule=bins)
+
+
+def test_malformed_stream_rejected():
+    g = L.gen_rmat(10, 8, seed=17)
+    cgr = g.encode(0, 128)
+    words = cgr.words.copy()
+    rec = np.diff(cgr.offsets.astype(np.int64))
+    i = int(np.argmax(rec))
+    mid = (int(cgr.offsets[i]) + int(rec[i]) // 2) // 32
+    words[mid:mid + 4] = 0      # 128 zero bits inside the longest record
+    e = L.Engine(0)
+    with pytest.raises(L.LpaError):
+        e.upload_arrays(cgr.n, cgr.n_c, words, cgr.nwords, cgr.offsets, cgr.vertices)
