@@ -117,6 +117,7 @@ def main():
         "kernel_ms_per_step": kt,
         "bins": {k: v for k, v in stats.items() if v["vertices"]},
         "changed": [int(x) for x in changed],
+        "device": e.info(),
         "wall_s": wall,
     }
     if not args.no_cglp:

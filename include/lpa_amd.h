@@ -90,6 +90,9 @@ int lpa_device_count(int *count);
 
 int lpa_engine_create(int device, lpa_engine **out);
 int lpa_engine_destroy(lpa_engine *e);
+/* info[0..3] = compute units, LDS bytes per workgroup, largest degree of the
+ * WAVE and of the BLOCK kernels' LDS tables. */
+int lpa_engine_info(lpa_engine *e, uint64_t *info);
 
 /* Graph in the reference's File layout: words are the big-endian 32-bit
  * words of File::graph (nwords of them), offsets the n_c+1 START bit offsets
@@ -108,6 +111,8 @@ int lpa_graph_degrees(lpa_engine *e, uint32_t *deg);
  * #residual segments, mean run length, log2(1+degree). */
 int lpa_graph_features(lpa_engine *e, float *feat);
 
+/* LP parameters; changing K or the CMS parameters drops the schedule, so
+ * call this before lpa_schedule_*. */
 int lpa_lp_config(lpa_engine *e, uint32_t K, int semi, uint32_t cms_thresh,
                   uint32_t cms_depth, uint32_t cms_width, uint64_t cms_seed);
 /* Degree-threshold schedule; CMS vertices (deg > cms_thresh) always go to

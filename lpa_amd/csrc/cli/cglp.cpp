@@ -1,0 +1,4 @@
+// cglp -- CLI driver (see driver.hpp).
+#include "driver.hpp"
+
+int main(int argc, char **argv) { return lpa_cli::driver_main(argc, argv, "cglp", 1); }

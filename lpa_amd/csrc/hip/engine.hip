@@ -16,11 +16,13 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <numeric>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
+#include "heavy_kernels.hpp"
 #include "lp_kernels.hpp"
 #include "lpa_amd.h"
 #include "prep_kernels.hpp"
@@ -83,6 +85,20 @@ template <class T> struct DevBuf {
     }
 };
 
+// Heavy vertices of one part (compressed or small CSR) and one mode (CMS or
+// exact HBM hash), processed by the chunked kernels of heavy_kernels.hpp.
+struct HeavySet {
+    bool csr_part = false;
+    bool cms = false;
+    uint32_t nh = 0, nchunks = 0, kb = 1;
+    DevBuf<uint32_t> pos, chunk_h;
+    DevBuf<uint64_t> loff, chunk_b, toff;
+    DevBuf<uint32_t> list;
+    DevBuf<uint64_t> tab;
+    DevBuf<uint32_t> sketch;
+    DevBuf<unsigned long long> result;
+};
+
 struct Bin {
     int kind;          // LPA_BIN_*
     int param;         // LANE: network size N; WAVE/BLOCK: log2 table slots
@@ -134,6 +150,9 @@ struct lpa_engine {
     bool scheduled = false;
     DevBuf<uint64_t> gtab;
     uint64_t gcap = 0;
+    DevBuf<uint32_t> glist;
+    uint64_t glcap = 0;
+    std::vector<std::unique_ptr<HeavySet>> heavy;
     uint32_t gblocks = 0;
 
     // LP state
@@ -184,12 +203,16 @@ struct lpa_engine {
         std::memcpy(a.cms_ab, cms_ab, sizeof cms_ab);
         a.gtab = gtab.p;
         a.gcap = gcap;
+        a.glist = glist.p;
+        a.glcap = glcap;
         a.changed = counter.p;
         return a;
     }
 
     void launch_bin(const Bin &b, StepArgs a, bool as_csr);
+    void launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr);
     void step(bool cglp, hipEvent_t *kind_events);
+    void run_heavy(bool cglp, hipEvent_t *kind_events, bool *started);
 };
 
 // ---------------------------------------------------------------- launches
@@ -204,6 +227,11 @@ template <bool CSR> static void launch_lane(int N, StepArgs a, uint32_t count, h
     }
 }
 
+// LDS bytes of the WAVE / BLOCK kernels' per-group table (2^logT slots) and
+// neighbour list (2^logT / 2 ids), for NG groups per workgroup.
+static size_t group_lds(int ng, uint32_t logT) { return (size_t)ng * 10ull * (1ull << logT); }
+static int wave_groups(uint32_t logT) { return logT <= 8 ? 16 : 4; }
+
 void lpa_engine::launch_bin(const Bin &b, StepArgs a, bool as_csr) {
     a.begin = b.begin;
     a.end = b.end;
@@ -215,36 +243,62 @@ void lpa_engine::launch_bin(const Bin &b, StepArgs a, bool as_csr) {
         else launch_lane<false>(b.param, a, count, stream);
         break;
     case LPA_BIN_WAVE: {
-        const uint32_t logT = (uint32_t)b.param;
-        const size_t lds = (size_t)(BLOCK / 64) * (8ull << logT);
-        dim3 grid((count + 3) / 4);
-        if (as_csr) hipLaunchKernelGGL((k_group<64, MODE_LDS, true>), grid, dim3(BLOCK), lds, stream, a, logT);
-        else hipLaunchKernelGGL((k_group<64, MODE_LDS, false>), grid, dim3(BLOCK), lds, stream, a, logT);
+        // a quarter wave per vertex up to degree 128, a whole wave above
+        const uint32_t logT = (uint32_t)b.param, cap = 1u << (logT - 1);
+        const int ng = wave_groups(logT);
+        const size_t lds = group_lds(ng, logT);
+        dim3 grid((count + ng - 1) / ng);
+        if (ng == 16) {
+            if (as_csr) hipLaunchKernelGGL((k_group<16, MODE_LDS, true>), grid, dim3(BLOCK), lds, stream, a, logT, cap);
+            else hipLaunchKernelGGL((k_group<16, MODE_LDS, false>), grid, dim3(BLOCK), lds, stream, a, logT, cap);
+        } else {
+            if (as_csr) hipLaunchKernelGGL((k_group<64, MODE_LDS, true>), grid, dim3(BLOCK), lds, stream, a, logT, cap);
+            else hipLaunchKernelGGL((k_group<64, MODE_LDS, false>), grid, dim3(BLOCK), lds, stream, a, logT, cap);
+        }
         break;
     }
     case LPA_BIN_BLOCK: {
-        const uint32_t logT = (uint32_t)b.param;
-        const size_t lds = 8ull << logT;
+        const uint32_t logT = (uint32_t)b.param, cap = 1u << (logT - 1);
+        const size_t lds = group_lds(1, logT);
         dim3 grid(count);
-        if (as_csr) hipLaunchKernelGGL((k_group<BLOCK, MODE_LDS, true>), grid, dim3(BLOCK), lds, stream, a, logT);
-        else hipLaunchKernelGGL((k_group<BLOCK, MODE_LDS, false>), grid, dim3(BLOCK), lds, stream, a, logT);
+        if (as_csr) hipLaunchKernelGGL((k_group<BLOCK, MODE_LDS, true>), grid, dim3(BLOCK), lds, stream, a, logT, cap);
+        else hipLaunchKernelGGL((k_group<BLOCK, MODE_LDS, false>), grid, dim3(BLOCK), lds, stream, a, logT, cap);
         break;
     }
-    case LPA_BIN_GLOBAL: {
-        const uint32_t logT = ceil_log2(2ull * b.maxdeg);
-        dim3 grid(std::min<uint32_t>(count, gblocks));
-        if (as_csr) hipLaunchKernelGGL((k_group<BLOCK, MODE_GLOBAL, true>), grid, dim3(BLOCK), 0, stream, a, logT);
-        else hipLaunchKernelGGL((k_group<BLOCK, MODE_GLOBAL, false>), grid, dim3(BLOCK), 0, stream, a, logT);
-        break;
+    default: break;   // CMS and GLOBAL bins run through launch_heavy
     }
-    case LPA_BIN_CMS: {
-        const size_t lds = (size_t)cms_depth * cms_width * sizeof(uint32_t);
-        dim3 grid(count);
-        if (as_csr) hipLaunchKernelGGL((k_group<BLOCK, MODE_CMS, true>), grid, dim3(BLOCK), lds, stream, a, 0u);
-        else hipLaunchKernelGGL((k_group<BLOCK, MODE_CMS, false>), grid, dim3(BLOCK), lds, stream, a, 0u);
-        break;
-    }
-    default: break;
+}
+
+void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
+    if (hs.nh == 0) return;
+    HeavyArgs h{};
+    h.s = a;
+    h.nh = hs.nh;
+    h.pos = hs.pos.p;
+    h.loff = hs.loff.p;
+    h.list = hs.list.p;
+    h.csr = as_csr;
+    h.toff = hs.toff.p;
+    h.tab = hs.tab.p;
+    h.sketch = hs.sketch.p;
+    h.result = hs.result.p;
+    h.nchunks = hs.nchunks;
+    h.chunk_h = hs.chunk_h.p;
+    h.chunk_b = hs.chunk_b.p;
+    h.cms = hs.cms ? 1 : 0;
+    if (!as_csr) hipLaunchKernelGGL(k_heavy_decode, dim3(hs.nh), dim3(BLOCK), 0, stream, h);
+    for (uint32_t k0 = 0; k0 < K; k0 += hs.kb) {
+        h.k0 = k0;
+        h.kb = std::min(hs.kb, K - k0);
+        hipLaunchKernelGGL(k_heavy_clear, dim3(hs.nh), dim3(BLOCK), 0, stream, h);
+        if (hs.cms) {
+            hipLaunchKernelGGL(k_heavy_chunks<1>, dim3(hs.nchunks), dim3(BLOCK), 0, stream, h);
+            hipLaunchKernelGGL(k_heavy_chunks<2>, dim3(hs.nchunks), dim3(BLOCK), 0, stream, h);
+        } else {
+            hipLaunchKernelGGL(k_heavy_chunks<0>, dim3(hs.nchunks), dim3(BLOCK), 0, stream, h);
+        }
+        const uint64_t fin = (uint64_t)hs.nh * h.kb;
+        hipLaunchKernelGGL(k_heavy_finish, dim3((unsigned)((fin + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, stream, h);
     }
 }
 
@@ -256,7 +310,7 @@ void lpa_engine::step(bool cglp, hipEvent_t *kind_events) {
     StepArgs a = base_args(cglp);
     bool started[LPA_NBIN_KINDS] = {false};
     for (const Bin &b : bins) {
-        if (b.kind == LPA_BIN_SKIP || b.end == b.begin) continue;
+        if (b.kind == LPA_BIN_SKIP || b.kind == LPA_BIN_CMS || b.kind == LPA_BIN_GLOBAL || b.end == b.begin) continue;
         if (kind_events && !started[b.kind]) {
             HIPCHK(hipEventRecord(kind_events[2 * b.kind], stream));
             started[b.kind] = true;
@@ -264,12 +318,30 @@ void lpa_engine::step(bool cglp, hipEvent_t *kind_events) {
         launch_bin(b, a, b.csr || cglp);
         if (kind_events) HIPCHK(hipEventRecord(kind_events[2 * b.kind + 1], stream));
     }
+    run_heavy(cglp, kind_events, started);
     HIPCHK(hipGetLastError());
+}
+
+void lpa_engine::run_heavy(bool cglp, hipEvent_t *kind_events, bool *started) {
+    for (auto &hp : heavy) {
+        HeavySet &hs = *hp;
+        if (hs.nh == 0) continue;
+        const int kind = hs.cms ? LPA_BIN_CMS : LPA_BIN_GLOBAL;
+        if (kind_events && !started[kind]) {
+            HIPCHK(hipEventRecord(kind_events[2 * kind], stream));
+            started[kind] = true;
+        }
+        const StepArgs a = base_args(cglp && !hs.csr_part);
+        launch_heavy(hs, a, hs.csr_part || cglp);
+        if (kind_events) HIPCHK(hipEventRecord(kind_events[2 * kind + 1], stream));
+    }
 }
 
 // ------------------------------------------------------------------ C ABI
 
 static uint64_t lds_budget(const lpa_engine *e);
+static uint32_t wave_cap(const lpa_engine *e);
+static uint32_t block_cap(const lpa_engine *e);
 
 extern "C" {
 
@@ -297,7 +369,9 @@ int lpa_engine_create(int device, lpa_engine **out) {
             e->counter.alloc(1);
             HIPCHK(hipHostMalloc((void **)&e->h_counter, sizeof(unsigned long long), hipHostMallocDefault));
             e->gblocks = (uint32_t)std::max(1, e->num_cu * 2);
-            const int dyn = (int)(e->lds_max - 8192);
+            const int dyn = (int)(e->lds_max - 20480);
+            (void)hipFuncSetAttribute((const void *)k_group<16, MODE_LDS, false>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+            (void)hipFuncSetAttribute((const void *)k_group<16, MODE_LDS, true>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
             (void)hipFuncSetAttribute((const void *)k_group<64, MODE_LDS, false>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
             (void)hipFuncSetAttribute((const void *)k_group<64, MODE_LDS, true>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
             (void)hipFuncSetAttribute((const void *)k_group<BLOCK, MODE_LDS, false>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
@@ -310,6 +384,16 @@ int lpa_engine_create(int device, lpa_engine **out) {
             throw;
         }
         *out = e;
+    });
+}
+
+int lpa_engine_info(lpa_engine *e, uint64_t *info) {
+    return guard([&] {
+        if (!e || !info) throw Fail(LPA_ERR_ARG, "null argument");
+        info[0] = (uint64_t)e->num_cu;
+        info[1] = (uint64_t)e->lds_max;
+        info[2] = wave_cap(e);
+        info[3] = block_cap(e);
     });
 }
 
@@ -433,7 +517,9 @@ int lpa_lp_config(lpa_engine *e, uint32_t K, int semi, uint32_t cms_thresh, uint
             if (cms_width == 0 || (cms_width & (cms_width - 1))) throw Fail(LPA_ERR_RANGE, "cms_width must be a power of two");
             if ((uint64_t)cms_depth * cms_width * 4 > lds_budget(e)) throw Fail(LPA_ERR_RANGE, "CMS sketch exceeds LDS");
         }
-        const bool changed_cms = (cms_thresh != e->cms_thresh);
+        // the schedule depends on K and the CMS parameters (bins, heavy-vertex buffers)
+        const bool changed_cms = cms_thresh != e->cms_thresh || K != e->K || cms_depth != e->cms_depth ||
+                                 cms_width != e->cms_width;
         e->K = K;
         e->semi = semi ? 1 : 0;
         e->cms_thresh = cms_thresh;
@@ -447,8 +533,62 @@ int lpa_lp_config(lpa_engine *e, uint32_t K, int semi, uint32_t cms_thresh, uint
             e->cms_ab[2 * r + 1] = splitmix64(s);
         }
         e->have_labels = false;
-        if (changed_cms) e->scheduled = false;   // CMS membership is part of the schedule
+        if (changed_cms) e->scheduled = false;   // call lpa_schedule_* again
     });
+}
+
+// Heavy sets from the CMS / GLOBAL bins: per heavy vertex its position,
+// list offsets (degree prefix), chunk table and, in exact mode, hash-table
+// offsets; tables of several label vectors share memory when K is large.
+static void build_heavy(lpa_engine *e) {
+    e->heavy.clear();
+    const uint64_t table_budget = 16ull << 30;   // bytes of HBM hash tables per batch of label vectors
+    for (int part = 0; part < 2; part++) {
+        for (int cms = 0; cms < 2; cms++) {
+            auto hs = std::make_unique<HeavySet>();
+            hs->csr_part = part == 1;
+            hs->cms = cms == 1;
+            std::vector<uint32_t> pos;
+            std::vector<uint64_t> loff(1, 0), toff(1, 0), cb;
+            std::vector<uint32_t> ch;
+            for (const Bin &b : e->bins) {
+                if (b.csr != hs->csr_part) continue;
+                if (b.kind != (cms ? LPA_BIN_CMS : LPA_BIN_GLOBAL)) continue;
+                for (uint32_t q = b.begin; q < b.end; q++) {
+                    const uint64_t d = hs->csr_part ? e->h_soffs[e->order_s[q] + 1] - e->h_soffs[e->order_s[q]]
+                                                    : e->deg_c[e->order_c[q]];
+                    const uint32_t i = (uint32_t)pos.size();
+                    pos.push_back(q);
+                    for (uint64_t c0 = 0; c0 < d; c0 += CHUNK) {
+                        ch.push_back(i);
+                        cb.push_back(c0);
+                    }
+                    loff.push_back(loff.back() + d);
+                    toff.push_back(toff.back() + (cms ? 0 : (1ull << ceil_log2(std::max<uint64_t>(2 * d, 2)))));
+                }
+            }
+            if (pos.empty()) continue;
+            hs->nh = (uint32_t)pos.size();
+            hs->nchunks = (uint32_t)ch.size();
+            hs->pos.upload(pos.data(), pos.size(), e->stream);
+            hs->loff.upload(loff.data(), loff.size(), e->stream);
+            hs->chunk_h.upload(ch.data(), ch.size(), e->stream);
+            hs->chunk_b.upload(cb.data(), cb.size(), e->stream);
+            if (!hs->csr_part) hs->list.alloc(loff.back());
+            if (hs->cms) {
+                hs->kb = e->K;
+                hs->sketch.alloc(((uint64_t)e->cms_depth * e->cms_width) * hs->nh * hs->kb);
+            } else {
+                const uint64_t per_vec = toff.back() * 8;
+                hs->kb = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(e->K, table_budget / std::max<uint64_t>(per_vec, 1)));
+                hs->toff.upload(toff.data(), toff.size(), e->stream);
+                hs->tab.alloc(toff.back() * hs->kb);
+            }
+            hs->result.alloc((uint64_t)hs->nh * hs->kb);
+            e->heavy.push_back(std::move(hs));
+        }
+    }
+    e->sync();
 }
 
 // Builds the device layout from a kind per file-order position (compressed
@@ -466,7 +606,7 @@ static void build_layout(lpa_engine *e, const std::vector<uint8_t> &kind_c, cons
     };
     e->bins.clear();
     std::memset(e->bin_counts, 0, sizeof e->bin_counts);
-    uint64_t gmax = 0;
+    uint64_t gmax = 0, lmax = 0;
     for (int part = 0; part < 2; part++) {
         const bool csr = part == 1;
         const uint64_t cnt = csr ? e->n_s : e->n_c;
@@ -491,6 +631,7 @@ static void build_layout(lpa_engine *e, const std::vector<uint8_t> &kind_c, cons
             e->bins.push_back(b);
             e->bin_counts[b.kind] += r - q;
             if (b.kind == LPA_BIN_GLOBAL) gmax = std::max<uint64_t>(gmax, b.maxdeg);
+            if ((b.kind == LPA_BIN_GLOBAL || b.kind == LPA_BIN_CMS) && !csr) lmax = std::max<uint64_t>(lmax, b.maxdeg);
             q = r;
         }
         if (!csr) {
@@ -530,6 +671,9 @@ static void build_layout(lpa_engine *e, const std::vector<uint8_t> &kind_c, cons
             e->sync();
         }
     }
+    build_heavy(e);
+    gmax = 0;
+    lmax = 0;
     if (gmax) {
         e->gcap = 1ull << ceil_log2(2 * gmax);
         e->gtab.alloc(e->gcap * e->gblocks);
@@ -537,23 +681,32 @@ static void build_layout(lpa_engine *e, const std::vector<uint8_t> &kind_c, cons
         e->gtab.release();
         e->gcap = 0;
     }
+    if (lmax) {
+        e->glcap = (lmax + 63) & ~63ull;
+        e->glist.alloc(e->glcap * e->gblocks);
+    } else {
+        e->glist.release();
+        e->glcap = 0;
+    }
     e->have_csr = false;
     e->scheduled = true;
     (void)p;
 }
 
-// LDS left for tables after the kernels' static shared memory
-static uint64_t lds_budget(const lpa_engine *e) { return e->lds_max - 8192; }
-// largest degree a WAVE (4 tables per workgroup) or BLOCK table can hold at load <= 1/2
+// LDS left for tables and lists after the kernels' static shared memory
+static uint64_t lds_budget(const lpa_engine *e) { return e->lds_max - 20480; }
+// largest degree the WAVE / BLOCK kernels' LDS can hold (table at load <= 1/2)
 static uint32_t wave_cap(const lpa_engine *e) {
-    uint32_t l = 0;
-    while ((4ull * 8ull << (l + 1)) <= lds_budget(e)) l++;
-    return std::min<uint32_t>(1u << (l - 1), 1024);
+    uint32_t best = 0;
+    for (uint32_t l = 6; l <= 11; l++)
+        if (group_lds(wave_groups(l), l) <= lds_budget(e)) best = 1u << (l - 1);
+    return best;
 }
 static uint32_t block_cap(const lpa_engine *e) {
-    uint32_t l = 0;
-    while ((8ull << (l + 1)) <= lds_budget(e)) l++;
-    return 1u << (l - 1);
+    uint32_t best = 0;
+    for (uint32_t l = 8; l <= 20; l++)
+        if (group_lds(1, l) <= lds_budget(e)) best = 1u << (l - 1);
+    return best;
 }
 
 static uint8_t legal_kind(const lpa_engine *e, int want, uint32_t d, const lpa_sched_params &p) {
@@ -561,7 +714,7 @@ static uint8_t legal_kind(const lpa_engine *e, int want, uint32_t d, const lpa_s
     if (e->cms_thresh && d > e->cms_thresh) return LPA_BIN_CMS;
     // promote to the first kind able to hold the degree
     if (want <= LPA_BIN_LANE && d <= 32) return LPA_BIN_LANE;
-    if (want <= LPA_BIN_WAVE && d <= std::min<uint32_t>(p.wave_max ? p.wave_max : 1024, wave_cap(e))) return LPA_BIN_WAVE;
+    if (want <= LPA_BIN_WAVE && d <= std::min<uint32_t>(p.wave_max ? p.wave_max : wave_cap(e), wave_cap(e))) return LPA_BIN_WAVE;
     if (want <= LPA_BIN_BLOCK && d <= block_cap(e)) return LPA_BIN_BLOCK;
     return LPA_BIN_GLOBAL;
 }
@@ -753,10 +906,12 @@ static void cglp_step(lpa_engine *e) {
     StepArgs a = e->base_args(false);
     StepArgs ac = e->base_args(true);
     for (const Bin &b : e->bins) {
-        if (b.kind == LPA_BIN_SKIP) continue;
+        if (b.kind == LPA_BIN_SKIP || b.kind == LPA_BIN_CMS || b.kind == LPA_BIN_GLOBAL) continue;
         if (b.csr) e->launch_bin(b, a, true);
         else e->launch_bin(b, ac, true);
     }
+    bool started[LPA_NBIN_KINDS] = {false};
+    e->run_heavy(true, nullptr, started);
     HIPCHK(hipGetLastError());
 }
 

@@ -56,9 +56,12 @@ struct StepArgs {
     // CMS (row hashes a_r, b_r)
     uint32_t cms_depth, cms_logw;
     uint64_t cms_ab[16];
-    // scratch hash tables in HBM: gcap entries per workgroup
+    // scratch in HBM for the largest degrees: per workgroup gcap table slots
+    // and glcap neighbour ids
     uint64_t *__restrict__ gtab;
     uint64_t gcap;
+    uint32_t *__restrict__ glist;
+    uint64_t glcap;
     unsigned long long *__restrict__ changed;
     uint32_t begin, end;                   // position range of this bin
 };
@@ -159,6 +162,8 @@ __global__ void __launch_bounds__(BLOCK) k_lane(StepArgs a) {
 
 enum { MODE_LDS = 0, MODE_GLOBAL = 1, MODE_CMS = 2 };
 
+// Groups of G lanes: G = 16 (four vertices per wave), 64 (one wave) or 256
+// (one workgroup).  Sub-wave groups synchronise with the wave.
 template <int G>
 __device__ __forceinline__ void group_sync() {
     if (G == BLOCK) {
@@ -170,9 +175,18 @@ __device__ __forceinline__ void group_sync() {
     }
 }
 
-__device__ __forceinline__ uint64_t wave_max_u64(uint64_t x) {
+// lanes of the wave that belong to this lane's group
+template <int G>
+__device__ __forceinline__ uint64_t group_mask() {
+    if (G >= 64) return ~0ull;
+    const uint32_t lane = __lane_id();
+    return (((1ull << G) - 1) << (lane & ~(uint32_t)(G - 1)));
+}
+
+template <int W>
+__device__ __forceinline__ uint64_t max_u64_over(uint64_t x) {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
+    for (int o = W / 2; o > 0; o >>= 1) {
         uint64_t y = __shfl_xor(x, o, 64);
         x = y > x ? y : x;
     }
@@ -198,15 +212,18 @@ __device__ __forceinline__ uint32_t tab_add(uint64_t *tab, uint32_t logT, uint32
     }
 }
 
-// Wave-level pre-aggregation: the active lanes holding the same label as the
-// first active lane are folded into one update by the lowest of them.
-// Returns the number of occurrences this lane must apply (0: folded away).
+// Pre-aggregation inside a group: the active lanes of the group holding the
+// same label as its first active lane are folded into one update by the
+// lowest of them.  Returns how many occurrences this lane applies (0: folded).
+template <int G>
 __device__ __forceinline__ uint32_t fold_first_label(uint32_t lab) {
-    const uint32_t first = __builtin_amdgcn_readfirstlane(lab);
-    const uint64_t m = __ballot(lab == first);
+    const uint64_t gm = group_mask<G>();
+    const uint64_t act = __ballot(1) & gm;
+    const int leader = __builtin_ctzll(act);
+    const uint32_t first = G >= 64 ? __builtin_amdgcn_readfirstlane(lab) : (uint32_t)__shfl((int)lab, leader, 64);
+    const uint64_t m = __ballot(lab == first) & gm;
     if (lab != first) return 1;
-    const uint32_t lane = __lane_id();
-    return lane == (uint32_t)__builtin_ctzll(m) ? (uint32_t)__popcll(m) : 0;
+    return __lane_id() == (uint32_t)__builtin_ctzll(m) ? (uint32_t)__popcll(m) : 0;
 }
 
 __device__ __forceinline__ uint32_t cms_col(const StepArgs &a, uint32_t r, uint32_t lab) {
@@ -214,32 +231,36 @@ __device__ __forceinline__ uint32_t cms_col(const StepArgs &a, uint32_t r, uint3
     return a.cms_logw ? (uint32_t)(h >> (64 - a.cms_logw)) : 0;
 }
 
-// Intervals at least this long are expanded by the whole group.
+// Intervals at least this long are written by the whole group.
 constexpr uint32_t LONG_RUN = 16;
+// Labels gathered per lane before they are counted.
+constexpr int UNROLL = 4;
 
 template <int G> struct GroupShared {
-    static constexpr int QCAP = G == BLOCK ? 512 : 64;
+    static constexpr int QCAP = G == BLOCK ? 512 : G == 64 ? 64 : 16;
     uint64_t res_start;
+    uint32_t cnt;                 // ids written to the list so far
     uint32_t qn;
     uint32_t qleft[QCAP];
     uint32_t qlen[QCAP];
+    uint32_t qpos[QCAP];
 };
 
-// Visit every neighbour of the vertex at position p with the G lanes of a
-// group: CSR rows are strided over lanes; CGR records are split into their
-// 256-bit segments, one per lane (interval block first, then the residual
-// block, which starts where the last interval segment ends).  Intervals of
-// LONG_RUN or more ids are queued and expanded by all lanes together, so a
-// hub whose neighbours form one long run is not walked by a single lane.
-template <int G, bool CSR, class F>
-__device__ __forceinline__ void group_neighbours(const StepArgs &a, uint32_t p, uint32_t v, uint32_t gl,
-                                                 GroupShared<G> *sh, F &&visit) {
-    if (CSR) {
-        const uint64_t b = a.row[p], e = a.row[p + 1];
-        for (uint64_t i = b + gl; i < e; i += G) visit(a.nbr[i]);
-        return;
+// Phase A: the G lanes of a group decode the record at position p into
+// `list` (order irrelevant to the mode).  Segments (256-bit units) go one
+// per lane: the interval block first, then the residual block, which starts
+// where the last interval segment ends.  Each run claims its slots with one
+// LDS atomic, each residual segment with one atomic for its whole count;
+// runs of LONG_RUN or more ids are written by all lanes together so a hub
+// whose neighbours form one long run is not written by a single lane.
+// Returns the degree (valid in every lane of the group).
+template <int G>
+__device__ __forceinline__ uint32_t group_decode(const StepArgs &a, uint32_t p, uint32_t v, uint32_t gl,
+                                                 GroupShared<G> *sh, uint32_t *list) {
+    if (gl == 0) {
+        sh->cnt = 0;
+        sh->qn = 0;
     }
-    if (gl == 0) sh->qn = 0;
     group_sync<G>();
     uint64_t base;
     const uint32_t ni = block_header(a.words, a.offs[p], &base);
@@ -247,43 +268,73 @@ __device__ __forceinline__ void group_neighbours(const StepArgs &a, uint32_t p, 
     for (uint32_t j = gl; j < ni; j += G) {
         uint64_t e = decode_interval_segment(a.words, base + (uint64_t)SEG_BITS * j, v,
                                              [&](uint32_t l, uint32_t len) {
+                                                 const uint32_t o = atomicAdd(&sh->cnt, len);
                                                  if (len >= LONG_RUN) {
-                                                     uint32_t q = atomicAdd(&sh->qn, 1u);
-                                                     if (q < GroupShared<G>::QCAP) {
+                                                     const uint32_t q = atomicAdd(&sh->qn, 1u);
+                                                     if (q < (uint32_t)GroupShared<G>::QCAP) {
                                                          sh->qleft[q] = l;
                                                          sh->qlen[q] = len;
+                                                         sh->qpos[q] = o;
                                                          return;
                                                      }
                                                  }
-                                                 for (uint32_t x = 0; x < len; x++) visit(l + x);
+                                                 for (uint32_t x = 0; x < len; x++) list[o + x] = l + x;
                                              });
         if (j == ni - 1) my_end = e;
     }
-    if (G == 64) {
-        if (gl == 0) sh->res_start = 0;   // unused for waves; keeps the slot defined
-    }
-    uint64_t rs;
-    if (G == 64) {
-        rs = __shfl(my_end, (int)((ni - 1) & 63), 64);
+    uint64_t rs = 0;
+    if (G <= 64) {
+        const int src = (int)((__lane_id() & ~(uint32_t)(G - 1)) + ((ni - 1) & (G - 1)));
+        rs = __shfl(my_end, src, 64);
     } else {
         if (gl == ((ni - 1) & (G - 1))) sh->res_start = my_end;
-    }
-    group_sync<G>();
-    if (G != 64) rs = sh->res_start;
-    const uint32_t nq = sh->qn < (uint32_t)GroupShared<G>::QCAP ? sh->qn : (uint32_t)GroupShared<G>::QCAP;
-    for (uint32_t q = 0; q < nq; q++) {
-        const uint32_t l = sh->qleft[q], len = sh->qlen[q];
-        for (uint32_t x = gl; x < len; x += G) visit(l + x);
+        group_sync<G>();
+        rs = sh->res_start;
     }
     const uint32_t nr = block_header(a.words, rs, &base);
-    for (uint32_t j = gl; j < nr; j += G)
-        decode_residual_segment(a.words, base + (uint64_t)SEG_BITS * j, v, visit);
+    for (uint32_t j = gl; j < nr; j += G) {
+        const uint64_t q = base + (uint64_t)SEG_BITS * j;
+        BitReader r;
+        r.seek(a.words, q);
+        uint32_t o = atomicAdd(&sh->cnt, r.gamma());
+        decode_residual_segment(a.words, q, v, [&](uint32_t u) { list[o++] = u; });
+    }
+    group_sync<G>();
+    const uint32_t nq = sh->qn < (uint32_t)GroupShared<G>::QCAP ? sh->qn : (uint32_t)GroupShared<G>::QCAP;
+    for (uint32_t q = 0; q < nq; q++) {
+        const uint32_t l = sh->qleft[q], len = sh->qlen[q], o = sh->qpos[q];
+        for (uint32_t x = gl; x < len; x += G) list[o + x] = l + x;
+    }
+    if (G == BLOCK) __threadfence_block();
+    group_sync<G>();
+    return sh->cnt;
 }
 
-// One group of G lanes per vertex.  LDS holds, per group, 2^logT table slots
-// (MODE_LDS), or the CMS sketch (MODE_CMS, per workgroup).
+// Phase B: lane gl of the group walks ids gl, gl+G, ... of the list,
+// gathering UNROLL labels before counting them.
+template <int G, class F>
+__device__ __forceinline__ void group_count(const StepArgs &a, const uint32_t *__restrict__ ids, uint32_t d,
+                                            uint32_t gl, uint32_t k, F &&use) {
+    for (uint32_t i0 = 0; i0 < d; i0 += UNROLL * G) {
+        uint32_t labs[UNROLL];
+#pragma unroll
+        for (int j = 0; j < UNROLL; j++) {
+            const uint32_t i = i0 + j * G + gl;
+            labs[j] = i < d ? clean_label(a.lin[(uint64_t)ids[i] * a.K + k]) : UNL;
+        }
+#pragma unroll
+        for (int j = 0; j < UNROLL; j++)
+            if (labs[j] != UNL) use(labs[j]);
+    }
+}
+
+// One group of G lanes per vertex.  Dynamic LDS holds, per group, the
+// neighbour list (listcap ids, MODE_LDS) followed by 2^logT table slots;
+// for MODE_GLOBAL both live in HBM scratch, for MODE_CMS the list lives in
+// HBM scratch and the sketch in LDS.  The record is decoded once and the K
+// label vectors are counted from the list.
 template <int G, int MODE, bool CSR>
-__global__ void __launch_bounds__(BLOCK) k_group(StepArgs a, uint32_t logT) {
+__global__ void __launch_bounds__(BLOCK) k_group(StepArgs a, uint32_t logT, uint32_t listcap) {
     constexpr int NG = BLOCK / G;
     extern __shared__ uint64_t lds[];
     __shared__ uint64_t red[BLOCK / 64];
@@ -291,33 +342,48 @@ __global__ void __launch_bounds__(BLOCK) k_group(StepArgs a, uint32_t logT) {
     const uint32_t gid = threadIdx.x / G, gl = threadIdx.x % G;
     const uint32_t T = 1u << logT;
     uint64_t *tab;
+    uint32_t *list;
     uint32_t *sketch = (uint32_t *)lds;
-    if (MODE == MODE_GLOBAL) tab = a.gtab + (uint64_t)blockIdx.x * a.gcap;
-    else tab = lds + (uint64_t)gid * T;
+    if (MODE == MODE_LDS) {
+        // per group: [table: T u64][list: listcap u32]
+        uint64_t *grp = lds + (uint64_t)gid * (T + (listcap + 1) / 2);
+        tab = grp;
+        list = (uint32_t *)(grp + T);
+    } else {
+        tab = a.gtab + (uint64_t)blockIdx.x * a.gcap;
+        list = a.glist + (uint64_t)blockIdx.x * a.glcap;
+    }
     GroupShared<G> *sh = &gsh[gid];
     const uint32_t stride = gridDim.x * NG;
-    for (uint32_t p = a.begin + blockIdx.x * NG + gid; ; p += stride) {
-        // every group of a workgroup runs the same trip count when G == BLOCK
-        if (p >= a.end) break;
-        const uint32_t v = CSR ? a.cvid[p] : a.vid[p];
+    for (uint32_t p0 = a.begin + blockIdx.x * NG; p0 < a.end; p0 += stride) {
+        // all groups of a wave (or workgroup) run the same trip count
+        const uint32_t p = p0 + gid;
+        const bool live = p < a.end;
+        const uint32_t v = live ? (CSR ? a.cvid[p] : a.vid[p]) : 0;
+        const uint32_t *ids = list;
+        uint32_t d = 0;
+        if (CSR) {
+            if (live) {
+                ids = a.nbr + a.row[p];
+                d = (uint32_t)(a.row[p + 1] - a.row[p]);
+            }
+        } else if (live) {
+            d = group_decode<G>(a, p, v, gl, sh, list);
+        }
         for (uint32_t k = 0; k < a.K; k++) {
             uint64_t best = 0;
             if (MODE == MODE_CMS) {
                 const uint32_t cells = a.cms_depth << a.cms_logw;
                 for (uint32_t i = gl; i < cells; i += G) sketch[i] = 0;
                 group_sync<G>();
-                group_neighbours<G, CSR>(a, p, v, gl, sh, [&](uint32_t u) {
-                    uint32_t lab = clean_label(a.lin[(uint64_t)u * a.K + k]);
-                    if (lab == UNL) return;
-                    const uint32_t cnt = fold_first_label(lab);
+                group_count<G>(a, ids, d, gl, k, [&](uint32_t lab) {
+                    const uint32_t cnt = fold_first_label<G>(lab);
                     if (cnt == 0) return;
                     for (uint32_t r = 0; r < a.cms_depth; r++)
                         atomicAdd(&sketch[(r << a.cms_logw) + cms_col(a, r, lab)], cnt);
                 });
                 group_sync<G>();
-                group_neighbours<G, CSR>(a, p, v, gl, sh, [&](uint32_t u) {
-                    uint32_t lab = clean_label(a.lin[(uint64_t)u * a.K + k]);
-                    if (lab == UNL) return;
+                group_count<G>(a, ids, d, gl, k, [&](uint32_t lab) {
                     uint32_t est = UNL;
                     for (uint32_t r = 0; r < a.cms_depth; r++) {
                         uint32_t c = sketch[(r << a.cms_logw) + cms_col(a, r, lab)];
@@ -327,34 +393,41 @@ __global__ void __launch_bounds__(BLOCK) k_group(StepArgs a, uint32_t logT) {
                     best = key > best ? key : best;
                 });
             } else {
-                for (uint32_t i = gl; i < T; i += G) tab[i] = EMPTY_SLOT;
+                // HBM tables are sized per vertex: the smallest power of two >= 2d
+                const uint32_t Tv = MODE == MODE_GLOBAL ? (d ? 1u << (32 - __builtin_clz(2 * d - 1)) : 1u) : T;
+                const uint32_t lt = MODE == MODE_GLOBAL ? 31 - __builtin_clz(Tv) : logT;
+                for (uint32_t i = gl; i < Tv; i += G) tab[i] = EMPTY_SLOT;
                 if (MODE == MODE_GLOBAL) __threadfence_block();
                 group_sync<G>();
-                group_neighbours<G, CSR>(a, p, v, gl, sh, [&](uint32_t u) {
-                    uint32_t lab = clean_label(a.lin[(uint64_t)u * a.K + k]);
-                    if (lab == UNL) return;
-                    const uint32_t cnt = fold_first_label(lab);
+                group_count<G>(a, ids, d, gl, k, [&](uint32_t lab) {
+                    const uint32_t cnt = fold_first_label<G>(lab);
                     if (cnt == 0) return;
-                    uint32_t c = tab_add(tab, logT, lab, cnt);
+                    uint32_t c = tab_add(tab, lt, lab, cnt);
                     uint64_t key = ((uint64_t)c << 32) | (uint32_t)~lab;
                     best = key > best ? key : best;
                 });
             }
-            best = wave_max_u64(best);
+            best = max_u64_over<(G < 64 ? G : 64)>(best);
             if (G == BLOCK) {
                 if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = best;
                 __syncthreads();
 #pragma unroll
                 for (int w = 0; w < BLOCK / 64; w++) best = red[w] > best ? red[w] : best;
             }
-            if (gl == 0) {
+            bool ch = false;
+            if (gl == 0 && live) {
                 uint64_t idx = (uint64_t)v * a.K + k;
                 uint32_t old = a.lin[idx];
                 uint32_t nl = pick(best, old);
                 a.lout[idx] = nl;
-                if (nl != old) atomicAdd(a.changed, 1ull);
+                ch = nl != old;
             }
-            group_sync<G>();   // table / sketch / red / queue reused next
+            if (G == BLOCK) {
+                if (ch) atomicAdd(a.changed, 1ull);
+            } else {
+                count_changes(ch, a.changed);
+            }
+            group_sync<G>();   // table / sketch / red reused next
         }
     }
 }

@@ -23,6 +23,7 @@ EXPORTS = {
     "lpa_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "lpa_engine_create": (C.c_int, [C.c_int, C.POINTER(_VP)]),
     "lpa_engine_destroy": (C.c_int, [_VP]),
+    "lpa_engine_info": (C.c_int, [_VP, _VP]),
     "lpa_graph_upload": (C.c_int, [_VP, C.c_uint64, C.c_uint64, _VP, C.c_uint64, _VP, _VP, _VP, _VP]),
     "lpa_graph_degrees": (C.c_int, [_VP, _VP]),
     "lpa_graph_features": (C.c_int, [_VP, _VP]),
@@ -99,6 +100,11 @@ class Engine:
 
     def __del__(self):
         self.close()
+
+    def info(self):
+        x = np.zeros(4, np.uint64)
+        _check(self._L.lpa_engine_info(self._h, _p(x)))
+        return dict(zip(("compute_units", "lds_bytes", "wave_cap", "block_cap"), map(int, x)))
 
     # graph -------------------------------------------------------------
     def upload(self, cgr: Cgr):
