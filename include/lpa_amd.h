@@ -143,6 +143,45 @@ int lpa_cglp_iterate(lpa_engine *e, uint64_t *changed);
 /* Copy of the device CSR by vertex id: row (n+1), col (m). */
 int lpa_cglp_csr_get(lpa_engine *e, uint64_t *row, uint32_t *col, uint64_t *m);
 
+/* ---- learned schedule (no reference counterpart) ----
+ * A per-vertex work recorder and an 8 -> 32 -> 32 -> 3 MLP cost model
+ * trained on the device.  The model predicts log2(1 + lane-cycles) of a
+ * vertex under each kernel family (0: wave-level kernels, 1: workgroup
+ * kernel with an LDS table, 2: chunked heavy-vertex kernels); the learned
+ * schedule bins each vertex into the family of least predicted cost.  It
+ * never changes results and never moves vertices in or out of CMS. */
+typedef struct lpa_mlp lpa_mlp;
+#define LPA_MLP_NPARAM 1443
+#define LPA_NFEAT 8
+
+/* Model inputs by vertex id (n x 8): log2(1 + x) of degree, #intervals,
+ * #residuals, encoded bits, #interval segments, #residual segments, mean
+ * interval length, and the previous-iteration change flag of label vector 0. */
+int lpa_model_features(lpa_engine *e, float *x);
+/* Runs one synchronous step with every vertex the schedule allows forced
+ * into kernel family `family` (0..2) and the recorder on: cost[v] receives
+ * the lane-cycles spent on v, measured[v] = 1 when v ran in that family.
+ * Labels and the previous schedule are restored afterwards. */
+int lpa_record_family(lpa_engine *e, int family, uint64_t *cost, uint8_t *measured);
+
+int lpa_mlp_create(int device, uint64_t seed, lpa_mlp **out);
+int lpa_mlp_destroy(lpa_mlp *m);
+int lpa_mlp_get_params(lpa_mlp *m, float *params);
+int lpa_mlp_set_params(lpa_mlp *m, const float *params);
+/* y (n x 3) = model(x (n x 8)); host arrays. */
+int lpa_mlp_forward(lpa_mlp *m, const float *x, uint64_t n, float *y);
+/* Mean over samples of the masked squared error and its gradient (1443). */
+int lpa_mlp_loss_grad(lpa_mlp *m, const float *x, const float *target, const uint8_t *mask, uint64_t n,
+                      float *loss, float *grad);
+/* Adam training on the device: epochs over shuffled mini-batches; loss of
+ * each epoch (mean over its batches) in loss_per_epoch. */
+int lpa_mlp_train(lpa_mlp *m, const float *x, const float *target, const uint8_t *mask, uint64_t n,
+                  int epochs, int batch, float lr, uint64_t seed, float *loss_per_epoch);
+/* Bins every vertex by the model (argmin of the predicted cost over the
+ * families that can hold its degree; CMS vertices stay CMS) and lays the
+ * graph out accordingly. */
+int lpa_schedule_learned(lpa_engine *e, lpa_mlp *m);
+
 /* Runs iters steps (method 0 = NOVA fused, 1 = CGLP) and reports the
  * HIP-event time of each (ms) and the changed counts; captures the step in
  * a HIP graph when use_graph != 0. */

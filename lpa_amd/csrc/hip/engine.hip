@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "heavy_kernels.hpp"
+#include "mlp_kernels.hpp"
 #include "lp_kernels.hpp"
 #include "lpa_amd.h"
 #include "prep_kernels.hpp"
@@ -175,6 +176,12 @@ struct lpa_engine {
     // timing of the last timed run, per bin kind
     float kind_ms[LPA_NBIN_KINDS] = {0};
 
+    // kind of every vertex under the current schedule (by vertex id)
+    std::vector<uint8_t> kind_of;
+    lpa_sched_params sched_used{32, 256, 0, 1};
+    // work recorder (lane-cycles per vertex id), null when off
+    unsigned long long *rec = nullptr;
+
     ~lpa_engine() {
         if (h_counter) (void)hipHostFree(h_counter);
         if (stream) (void)hipStreamDestroy(stream);
@@ -206,6 +213,7 @@ struct lpa_engine {
         a.glist = glist.p;
         a.glcap = glcap;
         a.changed = counter.p;
+        a.rec = rec;
         return a;
     }
 
@@ -292,10 +300,12 @@ void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
         h.kb = std::min(hs.kb, K - k0);
         hipLaunchKernelGGL(k_heavy_clear, dim3(hs.nh), dim3(BLOCK), 0, stream, h);
         if (hs.cms) {
-            hipLaunchKernelGGL(k_heavy_chunks<1>, dim3(hs.nchunks), dim3(BLOCK), 0, stream, h);
+            const size_t sk = (size_t)cms_depth * cms_width * sizeof(uint32_t);
+            hipLaunchKernelGGL(k_heavy_chunks<1>, dim3(hs.nchunks), dim3(BLOCK), sk, stream, h);
             hipLaunchKernelGGL(k_heavy_chunks<2>, dim3(hs.nchunks), dim3(BLOCK), 0, stream, h);
         } else {
-            hipLaunchKernelGGL(k_heavy_chunks<0>, dim3(hs.nchunks), dim3(BLOCK), 0, stream, h);
+            hipLaunchKernelGGL(k_heavy_chunks<0>, dim3(hs.nchunks), dim3(BLOCK), 2 * CHUNK * sizeof(uint64_t),
+                               stream, h);
         }
         const uint64_t fin = (uint64_t)hs.nh * h.kb;
         hipLaunchKernelGGL(k_heavy_finish, dim3((unsigned)((fin + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, stream, h);
@@ -378,6 +388,8 @@ int lpa_engine_create(int device, lpa_engine **out) {
             (void)hipFuncSetAttribute((const void *)k_group<BLOCK, MODE_LDS, true>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
             (void)hipFuncSetAttribute((const void *)k_group<BLOCK, MODE_CMS, false>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
             (void)hipFuncSetAttribute((const void *)k_group<BLOCK, MODE_CMS, true>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+            (void)hipFuncSetAttribute((const void *)k_heavy_chunks<0>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+            (void)hipFuncSetAttribute((const void *)k_heavy_chunks<1>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
             (void)hipGetLastError();
         } catch (...) {
             delete e;
@@ -595,6 +607,10 @@ static void build_heavy(lpa_engine *e) {
 // positions first, then small positions).
 static void build_layout(lpa_engine *e, const std::vector<uint8_t> &kind_c, const std::vector<uint8_t> &kind_s,
                          const lpa_sched_params &p) {
+    e->kind_of.assign(e->n, LPA_BIN_SKIP);
+    for (uint64_t i = 0; i < e->n_c; i++) e->kind_of[e->h_vertices[i]] = kind_c[i];
+    for (uint64_t j = 0; j < e->n_s; j++) e->kind_of[e->h_vertices[e->n_c + j]] = kind_s[j];
+    e->sched_used = p;
     auto lane_n = [](uint32_t d) { return d <= 4 ? 4 : d <= 8 ? 8 : d <= 16 ? 16 : 32; };
     auto sub_of = [&](int kind, uint32_t d) -> int {
         switch (kind) {
@@ -703,8 +719,9 @@ static uint32_t wave_cap(const lpa_engine *e) {
     return best;
 }
 static uint32_t block_cap(const lpa_engine *e) {
+    // up to 4096 slots (40 KB of table and list): several workgroups per CU
     uint32_t best = 0;
-    for (uint32_t l = 8; l <= 20; l++)
+    for (uint32_t l = 8; l <= 12; l++)
         if (group_lds(1, l) <= lds_budget(e)) best = 1u << (l - 1);
     return best;
 }
@@ -957,6 +974,289 @@ int lpa_cglp_csr_get(lpa_engine *e, uint64_t *row, uint32_t *col, uint64_t *m) {
             const uint32_t *s = kind[v] == 0 ? &ccol[crow[src[v]]] : &e->h_snbr[e->h_soffs[src[v]]];
             std::memcpy(col + row[v], s, deg[v] * 4);
         }
+    });
+}
+
+// ----------------------------------------------------- learned schedule
+
+struct lpa_mlp {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevBuf<float> params, grad, m1, m2, loss;
+    int t = 0;
+    ~lpa_mlp() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+static int family_of(int kind) {
+    switch (kind) {
+    case LPA_BIN_LANE:
+    case LPA_BIN_WAVE: return 0;
+    case LPA_BIN_BLOCK: return 1;
+    case LPA_BIN_GLOBAL: return 2;
+    default: return -1;
+    }
+}
+
+static uint32_t degree_of_vertex(const lpa_engine *e, std::vector<uint32_t> &deg) {
+    deg.assign(e->n, 0);
+    for (uint64_t i = 0; i < e->n_c; i++) deg[e->h_vertices[i]] = e->deg_c[i];
+    for (uint64_t j = 0; j < e->n_s; j++) deg[e->h_vertices[e->n_c + j]] = (uint32_t)(e->h_soffs[j + 1] - e->h_soffs[j]);
+    return 0;
+}
+
+// kinds per file-order position from kinds per vertex id
+static void split_kinds(const lpa_engine *e, const std::vector<uint8_t> &by_v, std::vector<uint8_t> &kc,
+                        std::vector<uint8_t> &ks) {
+    kc.resize(e->n_c);
+    ks.resize(e->n_s);
+    for (uint64_t i = 0; i < e->n_c; i++) kc[i] = by_v[e->h_vertices[i]];
+    for (uint64_t j = 0; j < e->n_s; j++) ks[j] = by_v[e->h_vertices[e->n_c + j]];
+}
+
+int lpa_model_features(lpa_engine *e, float *x) {
+    return guard([&] {
+        if (!e || !x) throw Fail(LPA_ERR_ARG, "null argument");
+        if (!e->have_graph) throw Fail(LPA_ERR_STATE, "no graph uploaded");
+        HIPCHK(hipSetDevice(e->device));
+        std::vector<float> raw(e->n * NFEAT);
+        {
+            int rc = lpa_graph_features(e, raw.data());
+            if (rc) throw Fail(rc, g_err);
+        }
+        std::vector<uint8_t> flag(e->n, 0);
+        if (e->have_labels) {
+            const uint64_t tot = e->n * e->K;
+            std::vector<uint32_t> a(tot), b(tot);
+            HIPCHK(hipMemcpy(a.data(), e->L[e->cur].p, tot * 4, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(b.data(), e->L[e->cur ^ 1].p, tot * 4, hipMemcpyDeviceToHost));
+            for (uint64_t v = 0; v < e->n; v++) flag[v] = a[v * e->K] != b[v * e->K];
+        }
+        for (uint64_t v = 0; v < e->n; v++) {
+            float *o = x + v * NFEAT;
+            const float *r = &raw[v * NFEAT];
+            for (int f = 0; f < 7; f++) o[f] = std::log2(1.0f + std::max(0.0f, r[f]));
+            o[7] = flag[v];
+        }
+    });
+}
+
+int lpa_record_family(lpa_engine *e, int family, uint64_t *cost, uint8_t *measured) {
+    return guard([&] {
+        require_ready(e);
+        if (family < 0 || family > 2 || !cost || !measured) throw Fail(LPA_ERR_ARG, "bad arguments");
+        const std::vector<uint8_t> saved_kind = e->kind_of;
+        const lpa_sched_params saved_p = e->sched_used;
+        const uint64_t tot = e->n * e->K;
+        std::vector<uint32_t> saved_lab(tot);
+        HIPCHK(hipMemcpy(saved_lab.data(), e->L[e->cur].p, tot * 4, hipMemcpyDeviceToHost));
+        const int saved_cur = e->cur;
+        // force the family where the kernel can hold the vertex
+        std::vector<uint32_t> deg;
+        degree_of_vertex(e, deg);
+        const int want = family == 0 ? LPA_BIN_LANE : family == 1 ? LPA_BIN_BLOCK : LPA_BIN_GLOBAL;
+        lpa_sched_params q{32, 0, 0, 1};
+        std::vector<uint8_t> byv(e->n);
+        for (uint64_t v = 0; v < e->n; v++) byv[v] = legal_kind(e, want, deg[v], q);
+        std::vector<uint8_t> kc, ks;
+        split_kinds(e, byv, kc, ks);
+        DevBuf<unsigned long long> rec;
+        try {
+            build_layout(e, kc, ks, q);
+            rec.alloc(e->n);
+            HIPCHK(hipMemsetAsync(rec.p, 0, e->n * 8, e->stream));
+            e->rec = rec.p;
+            e->step(false, nullptr);
+            e->rec = nullptr;
+            std::vector<unsigned long long> h(e->n);
+            HIPCHK(hipMemcpyAsync(h.data(), rec.p, e->n * 8, hipMemcpyDeviceToHost, e->stream));
+            e->sync();
+            for (uint64_t v = 0; v < e->n; v++) {
+                cost[v] = h[v];
+                measured[v] = family_of(byv[v]) == family;
+            }
+        } catch (...) {
+            e->rec = nullptr;
+            throw;
+        }
+        // restore labels and schedule
+        e->cur = saved_cur;
+        HIPCHK(hipMemcpy(e->L[e->cur].p, saved_lab.data(), tot * 4, hipMemcpyHostToDevice));
+        split_kinds(e, saved_kind, kc, ks);
+        build_layout(e, kc, ks, saved_p);
+    });
+}
+
+int lpa_mlp_create(int device, uint64_t seed, lpa_mlp **out) {
+    return guard([&] {
+        if (!out) throw Fail(LPA_ERR_ARG, "null argument");
+        auto *m = new lpa_mlp;
+        try {
+            m->device = device;
+            HIPCHK(hipSetDevice(device));
+            HIPCHK(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+            // He initialisation (Box-Muller over splitmix64), zero biases
+            std::vector<float> p(MLP_NPARAM, 0.0f);
+            uint64_t s = seed;
+            auto normal = [&]() {
+                double u1 = ((splitmix64(s) >> 11) + 1) * 0x1.0p-53, u2 = (splitmix64(s) >> 11) * 0x1.0p-53;
+                return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+            };
+            for (int i = 0; i < MLP_H * MLP_IN; i++) p[MLP_W1 + i] = (float)(normal() * std::sqrt(2.0 / MLP_IN));
+            for (int i = 0; i < MLP_H * MLP_H; i++) p[MLP_W2 + i] = (float)(normal() * std::sqrt(2.0 / MLP_H));
+            for (int i = 0; i < MLP_OUT * MLP_H; i++) p[MLP_W3 + i] = (float)(normal() * std::sqrt(2.0 / MLP_H));
+            m->params.upload(p.data(), MLP_NPARAM, m->stream);
+            m->grad.alloc(MLP_NPARAM);
+            m->m1.alloc(MLP_NPARAM);
+            m->m2.alloc(MLP_NPARAM);
+            m->loss.alloc(1);
+            HIPCHK(hipMemsetAsync(m->m1.p, 0, MLP_NPARAM * 4, m->stream));
+            HIPCHK(hipMemsetAsync(m->m2.p, 0, MLP_NPARAM * 4, m->stream));
+            HIPCHK(hipStreamSynchronize(m->stream));
+        } catch (...) {
+            delete m;
+            throw;
+        }
+        *out = m;
+    });
+}
+
+int lpa_mlp_destroy(lpa_mlp *m) {
+    return guard([&] {
+        if (!m) return;
+        (void)hipSetDevice(m->device);
+        delete m;
+    });
+}
+
+int lpa_mlp_get_params(lpa_mlp *m, float *params) {
+    return guard([&] {
+        if (!m || !params) throw Fail(LPA_ERR_ARG, "null argument");
+        HIPCHK(hipSetDevice(m->device));
+        HIPCHK(hipMemcpy(params, m->params.p, MLP_NPARAM * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+int lpa_mlp_set_params(lpa_mlp *m, const float *params) {
+    return guard([&] {
+        if (!m || !params) throw Fail(LPA_ERR_ARG, "null argument");
+        HIPCHK(hipSetDevice(m->device));
+        HIPCHK(hipMemcpy(m->params.p, params, MLP_NPARAM * 4, hipMemcpyHostToDevice));
+        m->t = 0;
+        HIPCHK(hipMemset(m->m1.p, 0, MLP_NPARAM * 4));
+        HIPCHK(hipMemset(m->m2.p, 0, MLP_NPARAM * 4));
+    });
+}
+
+int lpa_mlp_forward(lpa_mlp *m, const float *x, uint64_t n, float *y) {
+    return guard([&] {
+        if (!m || (n && (!x || !y))) throw Fail(LPA_ERR_ARG, "null argument");
+        if (n == 0) return;
+        HIPCHK(hipSetDevice(m->device));
+        DevBuf<float> dx, dy;
+        dx.upload(x, n * MLP_IN, m->stream);
+        dy.alloc(n * MLP_OUT);
+        hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->stream,
+                           m->params.p, dx.p, dy.p, n);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(y, dy.p, n * MLP_OUT * 4, hipMemcpyDeviceToHost, m->stream));
+        HIPCHK(hipStreamSynchronize(m->stream));
+    });
+}
+
+int lpa_mlp_loss_grad(lpa_mlp *m, const float *x, const float *target, const uint8_t *mask, uint64_t n,
+                      float *loss, float *grad) {
+    return guard([&] {
+        if (!m || !x || !target || !mask || !loss || !grad || n == 0) throw Fail(LPA_ERR_ARG, "bad arguments");
+        HIPCHK(hipSetDevice(m->device));
+        DevBuf<float> dx, dt;
+        DevBuf<uint8_t> dm;
+        dx.upload(x, n * MLP_IN, m->stream);
+        dt.upload(target, n * MLP_OUT, m->stream);
+        dm.upload(mask, n * MLP_OUT, m->stream);
+        hipLaunchKernelGGL(k_zero_f32, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, m->stream, m->grad.p, MLP_NPARAM);
+        hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(64), 0, m->stream, m->loss.p, 1);
+        hipLaunchKernelGGL(k_mlp_backward, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->stream, m->params.p,
+                           dx.p, dt.p, dm.p, (const uint32_t *)nullptr, n, 1.0f / (float)n, m->grad.p, m->loss.p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(grad, m->grad.p, MLP_NPARAM * 4, hipMemcpyDeviceToHost, m->stream));
+        HIPCHK(hipMemcpyAsync(loss, m->loss.p, 4, hipMemcpyDeviceToHost, m->stream));
+        HIPCHK(hipStreamSynchronize(m->stream));
+    });
+}
+
+int lpa_mlp_train(lpa_mlp *m, const float *x, const float *target, const uint8_t *mask, uint64_t n, int epochs,
+                  int batch, float lr, uint64_t seed, float *loss_per_epoch) {
+    return guard([&] {
+        if (!m || !x || !target || !mask || n == 0 || epochs <= 0 || batch <= 0) throw Fail(LPA_ERR_ARG, "bad arguments");
+        if (n >= (1ull << 32)) throw Fail(LPA_ERR_RANGE, "too many samples");
+        HIPCHK(hipSetDevice(m->device));
+        DevBuf<float> dx, dt;
+        DevBuf<uint8_t> dm;
+        DevBuf<uint32_t> didx;
+        dx.upload(x, n * MLP_IN, m->stream);
+        dt.upload(target, n * MLP_OUT, m->stream);
+        dm.upload(mask, n * MLP_OUT, m->stream);
+        didx.alloc(n);
+        std::vector<uint32_t> idx(n);
+        std::iota(idx.begin(), idx.end(), 0u);
+        uint64_t s = seed;
+        const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+        for (int ep = 0; ep < epochs; ep++) {
+            for (uint64_t i = n - 1; i > 0; i--) std::swap(idx[i], idx[splitmix64(s) % (i + 1)]);
+            HIPCHK(hipMemcpyAsync(didx.p, idx.data(), n * 4, hipMemcpyHostToDevice, m->stream));
+            hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(64), 0, m->stream, m->loss.p, 1);
+            int nb = 0;
+            for (uint64_t b0 = 0; b0 < n; b0 += (uint64_t)batch, nb++) {
+                const uint64_t bn = std::min<uint64_t>(batch, n - b0);
+                hipLaunchKernelGGL(k_zero_f32, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, m->stream, m->grad.p,
+                                   MLP_NPARAM);
+                hipLaunchKernelGGL(k_mlp_backward, dim3((unsigned)((bn + 255) / 256)), dim3(256), 0, m->stream,
+                                   m->params.p, dx.p, dt.p, dm.p, didx.p + b0, bn, 1.0f / (float)bn, m->grad.p,
+                                   m->loss.p);
+                m->t++;
+                const float c1 = (float)(1.0 / (1.0 - std::pow((double)b1, m->t)));
+                const float c2 = (float)(1.0 / (1.0 - std::pow((double)b2, m->t)));
+                hipLaunchKernelGGL(k_mlp_adam, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, m->stream, m->params.p,
+                                   m->grad.p, m->m1.p, m->m2.p, lr, b1, b2, eps, c1, c2);
+            }
+            HIPCHK(hipGetLastError());
+            float l = 0;
+            HIPCHK(hipMemcpyAsync(&l, m->loss.p, 4, hipMemcpyDeviceToHost, m->stream));
+            HIPCHK(hipStreamSynchronize(m->stream));
+            if (loss_per_epoch) loss_per_epoch[ep] = l / (float)nb;
+        }
+    });
+}
+
+int lpa_schedule_learned(lpa_engine *e, lpa_mlp *m) {
+    return guard([&] {
+        if (!e || !m) throw Fail(LPA_ERR_ARG, "null argument");
+        if (!e->have_graph) throw Fail(LPA_ERR_STATE, "no graph uploaded");
+        std::vector<float> x(e->n * NFEAT), y(e->n * MLP_OUT);
+        int rc = lpa_model_features(e, x.data());
+        if (rc) throw Fail(rc, g_err);
+        rc = lpa_mlp_forward(m, x.data(), e->n, y.data());
+        if (rc) throw Fail(rc, g_err);
+        HIPCHK(hipSetDevice(e->device));
+        std::vector<uint32_t> deg;
+        degree_of_vertex(e, deg);
+        lpa_sched_params q{32, 0, 0, 1};
+        const uint32_t wc = wave_cap(e), bc = block_cap(e);
+        std::vector<uint8_t> byv(e->n);
+        for (uint64_t v = 0; v < e->n; v++) {
+            const uint32_t d = deg[v];
+            int fam = 2;
+            float best = y[v * MLP_OUT + 2];
+            if (d <= bc && y[v * MLP_OUT + 1] < best) { fam = 1; best = y[v * MLP_OUT + 1]; }
+            if (d <= wc && y[v * MLP_OUT + 0] < best) { fam = 0; best = y[v * MLP_OUT + 0]; }
+            const int want = fam == 0 ? LPA_BIN_LANE : fam == 1 ? LPA_BIN_BLOCK : LPA_BIN_GLOBAL;
+            byv[v] = legal_kind(e, want, d, q);
+        }
+        std::vector<uint8_t> kc, ks;
+        split_kinds(e, byv, kc, ks);
+        build_layout(e, kc, ks, q);
     });
 }
 

@@ -28,7 +28,7 @@
 
 namespace lpa {
 
-constexpr uint32_t CHUNK = 2048;
+constexpr uint32_t CHUNK = 4096;
 
 struct HeavyArgs {
     StepArgs s;                                // words/offs/vid, row/nbr/cvid, labels, CMS hashes
@@ -61,8 +61,10 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_decode(HeavyArgs h) {
     __shared__ GroupShared<BLOCK> sh;
     const uint32_t i = blockIdx.x;
     if (i >= h.nh || h.csr) return;
+    const uint64_t t0 = h.s.rec ? cycles() : 0;
     const uint32_t p = h.pos[i];
     group_decode<BLOCK>(h.s, p, h.s.vid[p], threadIdx.x, &sh, h.list + h.loff[i]);
+    if (h.s.rec && threadIdx.x == 0) atomicAdd(&h.s.rec[h.s.vid[p]], (cycles() - t0) * BLOCK);
 }
 
 __global__ void __launch_bounds__(BLOCK) k_heavy_clear(HeavyArgs h) {
@@ -78,9 +80,15 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_clear(HeavyArgs h) {
     }
 }
 
-// PASS 1: sketch increments, 2: sketch queries, 0: exact counting
+// PASS 1: sketch increments, 2: sketch queries, 0: exact counting.
+// Passes 1 and 0 first aggregate the chunk in LDS (a chunk-local sketch, or
+// a chunk-local hash table of 2*CHUNK slots) and then merge only the
+// non-zero cells / occupied slots into the vertex's HBM sketch or table, so
+// a hub whose neighbours share a few labels does not funnel every arc into
+// the same HBM atomic.  Dynamic LDS: the local sketch or table.
 template <int PASS>
 __global__ void __launch_bounds__(BLOCK) k_heavy_chunks(HeavyArgs h) {
+    extern __shared__ uint64_t lds[];
     __shared__ uint64_t red[BLOCK / 64];
     const uint32_t c = blockIdx.x;
     if (c >= h.nchunks) return;
@@ -92,19 +100,24 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_chunks(HeavyArgs h) {
     const StepArgs &a = h.s;
     const uint64_t T = h.cms ? 0 : h.toff[i + 1] - h.toff[i];
     const uint32_t logT = h.cms ? 0 : 63 - __builtin_clzll(T);
-    const uint64_t cells = (uint64_t)a.cms_depth << a.cms_logw;
+    const uint32_t cells = a.cms_depth << a.cms_logw;
+    constexpr uint32_t LT = 2 * CHUNK, LOG_LT = 31 - __builtin_clz(LT);
+    uint32_t *lsk = (uint32_t *)lds;   // PASS 1
+    uint64_t *ltab = lds;              // PASS 0
+    const uint64_t t0 = a.rec ? cycles() : 0;
     for (uint32_t kk = 0; kk < h.kb; kk++) {
         const uint32_t k = h.k0 + kk;
         uint64_t best = 0;
         uint32_t *sk = h.cms ? h.sketch + ((uint64_t)i * h.kb + kk) * cells : nullptr;
         uint64_t *tab = h.cms ? nullptr : h.tab + h.toff[i] * h.kb + kk * T;
+        if (PASS == 1) {
+            for (uint32_t x = threadIdx.x; x < cells; x += BLOCK) lsk[x] = 0;
+        } else if (PASS == 0) {
+            for (uint32_t x = threadIdx.x; x < LT; x += BLOCK) ltab[x] = EMPTY_SLOT;
+        }
+        __syncthreads();
         group_count<BLOCK>(a, ids, n, threadIdx.x, k, [&](uint32_t lab) {
-            if (PASS == 1) {
-                const uint32_t cnt = fold_first_label<BLOCK>(lab);
-                if (cnt == 0) return;
-                for (uint32_t r = 0; r < a.cms_depth; r++)
-                    atomicAdd(&sk[(r << a.cms_logw) + cms_col(a, r, lab)], cnt);
-            } else if (PASS == 2) {
+            if (PASS == 2) {
                 uint32_t est = UNL;
                 for (uint32_t r = 0; r < a.cms_depth; r++) {
                     uint32_t x = sk[(r << a.cms_logw) + cms_col(a, r, lab)];
@@ -112,14 +125,31 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_chunks(HeavyArgs h) {
                 }
                 uint64_t key = ((uint64_t)est << 32) | (uint32_t)~lab;
                 best = key > best ? key : best;
+                return;
+            }
+            const uint32_t cnt = fold_first_label<BLOCK>(lab);
+            if (cnt == 0) return;
+            if (PASS == 1) {
+                for (uint32_t r = 0; r < a.cms_depth; r++)
+                    atomicAdd(&lsk[(r << a.cms_logw) + cms_col(a, r, lab)], cnt);
             } else {
-                const uint32_t cnt = fold_first_label<BLOCK>(lab);
-                if (cnt == 0) return;
-                uint32_t cc = tab_add(tab, logT, lab, cnt);
+                (void)tab_add(ltab, LOG_LT, lab, cnt);
+            }
+        });
+        __syncthreads();
+        if (PASS == 1) {
+            for (uint32_t x = threadIdx.x; x < cells; x += BLOCK)
+                if (lsk[x]) atomicAdd(&sk[x], lsk[x]);
+        } else if (PASS == 0) {
+            for (uint32_t x = threadIdx.x; x < LT; x += BLOCK) {
+                const uint64_t e = ltab[x];
+                if ((uint32_t)(e >> 32) == UNL) continue;
+                const uint32_t lab = (uint32_t)(e >> 32);
+                const uint32_t cc = tab_add(tab, logT, lab, (uint32_t)e);
                 uint64_t key = ((uint64_t)cc << 32) | (uint32_t)~lab;
                 best = key > best ? key : best;
             }
-        });
+        }
         if (PASS != 1) {
             best = max_u64_over<64>(best);
             if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = best;
@@ -128,8 +158,12 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_chunks(HeavyArgs h) {
                 for (int w = 0; w < BLOCK / 64; w++) best = red[w] > best ? red[w] : best;
                 if (best) atomicMax(&h.result[(uint64_t)i * h.kb + kk], (unsigned long long)best);
             }
-            __syncthreads();
         }
+        __syncthreads();
+    }
+    if (a.rec && threadIdx.x == 0) {
+        const uint32_t p = h.pos[i];
+        atomicAdd(&a.rec[h.csr ? a.cvid[p] : a.vid[p]], (cycles() - t0) * BLOCK);
     }
 }
 

@@ -64,7 +64,12 @@ struct StepArgs {
     uint64_t glcap;
     unsigned long long *__restrict__ changed;
     uint32_t begin, end;                   // position range of this bin
+    // work recorder (null: off): lane-cycles spent per vertex id
+    unsigned long long *__restrict__ rec;
 };
+
+// shader clock (cycles)
+__device__ __forceinline__ uint64_t cycles() { return __builtin_readcyclecounter(); }
 
 __device__ __forceinline__ uint32_t clean_label(uint32_t raw) {
     return raw == UNL ? UNL : (raw & ~SEED_BIT);
@@ -121,6 +126,7 @@ __global__ void __launch_bounds__(BLOCK) k_lane(StepArgs a) {
     __shared__ uint32_t list[CSR ? 1 : N][BLOCK];
     const uint32_t p = a.begin + blockIdx.x * BLOCK + threadIdx.x;
     const bool active = p < a.end;
+    const uint64_t t0 = a.rec ? cycles() : 0;
     uint32_t v = 0, d = 0;
     uint64_t base = 0;
     if (active) {
@@ -156,6 +162,7 @@ __global__ void __launch_bounds__(BLOCK) k_lane(StepArgs a) {
         }
         count_changes(ch, a.changed);
     }
+    if (a.rec && active) a.rec[v] = cycles() - t0;
 }
 
 // ------------------------------------------------------------ group kernels
@@ -234,7 +241,10 @@ __device__ __forceinline__ uint32_t cms_col(const StepArgs &a, uint32_t r, uint3
 // Intervals at least this long are written by the whole group.
 constexpr uint32_t LONG_RUN = 16;
 // Labels gathered per lane before they are counted.
-constexpr int UNROLL = 4;
+#ifndef LPA_UNROLL
+#define LPA_UNROLL 4
+#endif
+constexpr int UNROLL = LPA_UNROLL;
 
 template <int G> struct GroupShared {
     static constexpr int QCAP = G == BLOCK ? 512 : G == 64 ? 64 : 16;
@@ -292,12 +302,20 @@ __device__ __forceinline__ uint32_t group_decode(const StepArgs &a, uint32_t p, 
         rs = sh->res_start;
     }
     const uint32_t nr = block_header(a.words, rs, &base);
+    // the reader of segment j+G is loaded before segment j is decoded, so a
+    // lane with several segments overlaps the next load with decoding
+    BitReader r;
+    if (gl < nr) r.seek(a.words, base + (uint64_t)SEG_BITS * gl);
     for (uint32_t j = gl; j < nr; j += G) {
-        const uint64_t q = base + (uint64_t)SEG_BITS * j;
-        BitReader r;
-        r.seek(a.words, q);
-        uint32_t o = atomicAdd(&sh->cnt, r.gamma());
-        decode_residual_segment(a.words, q, v, [&](uint32_t u) { list[o++] = u; });
+        BitReader cur = r;
+        if (j + G < nr) r.seek(a.words, base + (uint64_t)SEG_BITS * (j + G));
+        const uint32_t c = cur.gamma();
+        uint32_t o = atomicAdd(&sh->cnt, c);
+        uint32_t u = 0;
+        for (uint32_t t = 0; t < c; t++) {
+            u = (t == 0) ? from_nat(v, cur.zeta()) : u + cur.zeta() + 1;
+            list[o++] = u;
+        }
     }
     group_sync<G>();
     const uint32_t nq = sh->qn < (uint32_t)GroupShared<G>::QCAP ? sh->qn : (uint32_t)GroupShared<G>::QCAP;
@@ -359,6 +377,7 @@ __global__ void __launch_bounds__(BLOCK) k_group(StepArgs a, uint32_t logT, uint
         // all groups of a wave (or workgroup) run the same trip count
         const uint32_t p = p0 + gid;
         const bool live = p < a.end;
+        const uint64_t t0 = a.rec ? cycles() : 0;
         const uint32_t v = live ? (CSR ? a.cvid[p] : a.vid[p]) : 0;
         const uint32_t *ids = list;
         uint32_t d = 0;
@@ -429,6 +448,7 @@ __global__ void __launch_bounds__(BLOCK) k_group(StepArgs a, uint32_t logT, uint
             }
             group_sync<G>();   // table / sketch / red reused next
         }
+        if (a.rec && live && gl == 0) a.rec[v] = (cycles() - t0) * G;
     }
 }
 

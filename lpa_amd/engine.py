@@ -39,6 +39,16 @@ EXPORTS = {
     "lpa_cglp_iterate": (C.c_int, [_VP, C.POINTER(C.c_uint64)]),
     "lpa_cglp_csr_get": (C.c_int, [_VP, _VP, _VP, C.POINTER(C.c_uint64)]),
     "lpa_time_iterations": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, _VP, _VP]),
+    "lpa_model_features": (C.c_int, [_VP, _VP]),
+    "lpa_record_family": (C.c_int, [_VP, C.c_int, _VP, _VP]),
+    "lpa_mlp_create": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(_VP)]),
+    "lpa_mlp_destroy": (C.c_int, [_VP]),
+    "lpa_mlp_get_params": (C.c_int, [_VP, _VP]),
+    "lpa_mlp_set_params": (C.c_int, [_VP, _VP]),
+    "lpa_mlp_forward": (C.c_int, [_VP, _VP, C.c_uint64, _VP]),
+    "lpa_mlp_loss_grad": (C.c_int, [_VP, _VP, _VP, _VP, C.c_uint64, _VP, _VP]),
+    "lpa_mlp_train": (C.c_int, [_VP, _VP, _VP, _VP, C.c_uint64, C.c_int, C.c_int, C.c_float, C.c_uint64, _VP]),
+    "lpa_schedule_learned": (C.c_int, [_VP, _VP]),
     "lpa_last_kernel_times": (C.c_int, [_VP, _VP]),
 }
 
@@ -200,7 +210,80 @@ class Engine:
                                            int(use_graph), _p(ms), _p(ch)))
         return ms, ch
 
+    # learned schedule ------------------------------------------------------
+    def model_features(self):
+        x = np.zeros((self.n, 8), np.float32)
+        _check(self._L.lpa_model_features(self._h, _p(x)))
+        return x
+
+    def record_family(self, family):
+        """one step with every vertex forced into kernel family 0/1/2 and the
+        recorder on: (lane-cycles per vertex, measured mask)"""
+        cost = np.zeros(self.n, np.uint64)
+        meas = np.zeros(self.n, np.uint8)
+        _check(self._L.lpa_record_family(self._h, family, _p(cost), _p(meas)))
+        return cost, meas.astype(bool)
+
+    def schedule_learned(self, mlp):
+        _check(self._L.lpa_schedule_learned(self._h, mlp._h))
+
     def kernel_times(self):
         t = np.zeros(6, np.float32)
         _check(self._L.lpa_last_kernel_times(self._h, _p(t)))
         return dict(zip(BIN_NAMES, t.tolist()))
+
+
+MLP_NPARAM = 1443
+
+
+class Mlp:
+    """8 -> 32 -> 32 -> 3 cost model with device forward / backward / Adam."""
+
+    def __init__(self, device=0, seed=0):
+        self._L = load()
+        h = _VP()
+        _check(self._L.lpa_mlp_create(device, seed, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.lpa_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def params(self):
+        p = np.zeros(MLP_NPARAM, np.float32)
+        _check(self._L.lpa_mlp_get_params(self._h, _p(p)))
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.shape == (MLP_NPARAM,)
+        _check(self._L.lpa_mlp_set_params(self._h, _p(p)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros((len(x), 3), np.float32)
+        _check(self._L.lpa_mlp_forward(self._h, _p(x), len(x), _p(y)))
+        return y
+
+    def loss_grad(self, x, t, mask):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        m = np.ascontiguousarray(mask, np.uint8)
+        loss = np.zeros(1, np.float32)
+        g = np.zeros(MLP_NPARAM, np.float32)
+        _check(self._L.lpa_mlp_loss_grad(self._h, _p(x), _p(t), _p(m), len(x), _p(loss), _p(g)))
+        return float(loss[0]), g
+
+    def train(self, x, t, mask, epochs=20, batch=1024, lr=1e-3, seed=0):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        m = np.ascontiguousarray(mask, np.uint8)
+        losses = np.zeros(epochs, np.float32)
+        _check(self._L.lpa_mlp_train(self._h, _p(x), _p(t), _p(m), len(x), epochs, batch, lr, seed, _p(losses)))
+        return losses

@@ -1,0 +1,64 @@
+"""Learned schedule: the device MLP against the float64 CPU reference, and
+labels that do not depend on the schedule."""
+import numpy as np
+import pytest
+
+import lpa_amd as L
+import mlp_ref as R
+from gpu_common import run_and_compare
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mlp_forward_matches_reference():
+    m = L.engine.Mlp(0, seed=3)
+    rng = np.random.default_rng(0)
+    p = rng.normal(0, 0.2, R.NPARAM).astype(np.float32)
+    m.params = p
+    x = rng.normal(0, 2, (5000, 8)).astype(np.float32)
+    y = m.forward(x)
+    want, _ = R.forward(p.astype(np.float64), x.astype(np.float64))
+    assert np.allclose(y, want, rtol=1e-4, atol=1e-4)
+
+
+def test_mlp_gradient_matches_reference():
+    m = L.engine.Mlp(0, seed=4)
+    rng = np.random.default_rng(1)
+    p = rng.normal(0, 0.2, R.NPARAM).astype(np.float32)
+    m.params = p
+    x = rng.normal(0, 1, (3000, 8)).astype(np.float32)
+    t = rng.normal(0, 1, (3000, 3)).astype(np.float32)
+    mask = rng.random((3000, 3)) < 0.6
+    loss, g = m.loss_grad(x, t, mask)
+    want_l, want_g = R.loss_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64), mask)
+    assert abs(loss - want_l) <= 1e-4 * abs(want_l)
+    scale = np.abs(want_g).max()
+    assert np.allclose(g, want_g, rtol=1e-3, atol=1e-5 * scale)
+
+
+def test_mlp_training_reduces_loss():
+    rng = np.random.default_rng(2)
+    x = rng.uniform(0, 10, (20000, 8)).astype(np.float32)
+    t = np.stack([x[:, 0] * 0.5 + 1, np.abs(x[:, 1] - x[:, 3]), 0.2 * x[:, 2] * x[:, 4] / 10], 1).astype(np.float32)
+    mask = np.ones_like(t, np.uint8)
+    m = L.engine.Mlp(0, seed=5)
+    losses = m.train(x, t, mask, epochs=30, batch=512, lr=3e-3)
+    assert losses[-1] < 0.1 * losses[0]
+    # the recorded loss is the reference loss of the trained parameters
+    final = R.loss(m.params.astype(np.float64), x.astype(np.float64), t.astype(np.float64), mask.astype(bool))
+    assert final < 0.2 * losses[0]
+
+
+def test_learned_schedule_is_bit_exact():
+    g = L.gen_rmat(14, 16, seed=21)
+    cgr = g.encode(0, 128)
+    e = L.Engine(0)
+    e.upload(cgr)
+    e.config(K=2, cms_thresh=2000, cms_depth=4, cms_width=512, cms_seed=1)
+    e.schedule_degree()
+    e.labels_init()
+    samples = L.schedule.record_samples(e)
+    model, losses = L.schedule.train_cost_model([samples], epochs=15, batch=1024)
+    assert np.isfinite(losses).all() and losses[-1] < losses[0]
+    e.schedule_learned(model)
+    run_and_compare(cgr, 4, K=2, cms_thresh=2000, cms_depth=4, cms_width=512, cms_seed=1, engine=e)
