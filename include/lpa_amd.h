@@ -81,8 +81,9 @@ enum {
 typedef struct {
     uint32_t lane_max;      /* deg <= lane_max -> LANE (<= 32) */
     uint32_t wave_max;      /* deg <= wave_max -> WAVE */
-    uint32_t block_max;     /* deg <= block_max -> BLOCK, else GLOBAL */
-    uint32_t relayout;      /* 1: reorder records on device so each bin is contiguous */
+    uint32_t block_max;     /* deg <= block_max -> BLOCK, else the heavy-vertex path (0: LDS limit) */
+    uint32_t relayout;      /* records are always reordered so each bin is contiguous; keep 1 */
+    uint32_t quarter_max;   /* WAVE vertices up to this degree use 16-lane groups (0: 128) */
 } lpa_sched_params;
 
 const char *lpa_last_error(void);
@@ -187,8 +188,15 @@ int lpa_schedule_learned(lpa_engine *e, lpa_mlp *m);
  * a HIP graph when use_graph != 0. */
 int lpa_time_iterations(lpa_engine *e, int method, int iters, int use_graph,
                         float *ms, uint64_t *changed);
-/* Average duration (ms) of the dominant kernel of the last timed run and
- * its name (HIP events on the engine stream). */
+/* Replays steps from HIP graphs captured once per label-buffer parity
+ * (on != 0), or launches them eagerly (0, the default). */
+int lpa_set_graph_mode(lpa_engine *e, int on);
+/* Times `iters` fused steps for each candidate schedule (lane, wave and
+ * quarter-wave thresholds), keeps the fastest (median), and restores the
+ * labels after every candidate, so tuning never changes results. */
+int lpa_autotune(lpa_engine *e, int iters, lpa_sched_params *best, float *best_ms);
+/* Average time (ms) per step of each kernel family in the last eager timed
+ * run (HIP events on the engine stream). */
 int lpa_last_kernel_times(lpa_engine *e, float *ms_per_kind /* LPA_NBIN_KINDS */);
 
 #ifdef __cplusplus

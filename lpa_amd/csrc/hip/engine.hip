@@ -182,7 +182,20 @@ struct lpa_engine {
     // work recorder (lane-cycles per vertex id), null when off
     unsigned long long *rec = nullptr;
 
+    // captured steps: graphs[method][cur]
+    bool graph_mode = false;
+    hipGraphExec_t graphs[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    void drop_graphs() {
+        for (auto &m : graphs)
+            for (auto &g : m)
+                if (g) {
+                    (void)hipGraphExecDestroy(g);
+                    g = nullptr;
+                }
+    }
+
     ~lpa_engine() {
+        drop_graphs();
         if (h_counter) (void)hipHostFree(h_counter);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -238,7 +251,9 @@ template <bool CSR> static void launch_lane(int N, StepArgs a, uint32_t count, h
 // LDS bytes of the WAVE / BLOCK kernels' per-group table (2^logT slots) and
 // neighbour list (2^logT / 2 ids), for NG groups per workgroup.
 static size_t group_lds(int ng, uint32_t logT) { return (size_t)ng * 10ull * (1ull << logT); }
-static int wave_groups(uint32_t logT) { return logT <= 8 ? 16 : 4; }
+static int wave_groups(uint32_t logT, uint32_t quarter_max = 128) {
+    return (1u << (logT - 1)) <= quarter_max ? 16 : 4;
+}
 
 void lpa_engine::launch_bin(const Bin &b, StepArgs a, bool as_csr) {
     a.begin = b.begin;
@@ -253,7 +268,7 @@ void lpa_engine::launch_bin(const Bin &b, StepArgs a, bool as_csr) {
     case LPA_BIN_WAVE: {
         // a quarter wave per vertex up to degree 128, a whole wave above
         const uint32_t logT = (uint32_t)b.param, cap = 1u << (logT - 1);
-        const int ng = wave_groups(logT);
+        const int ng = wave_groups(logT, sched_used.quarter_max ? sched_used.quarter_max : 128);
         const size_t lds = group_lds(ng, logT);
         dim3 grid((count + ng - 1) / ng);
         if (ng == 16) {
@@ -427,6 +442,7 @@ int lpa_graph_upload(lpa_engine *e, uint64_t n, uint64_t n_c, const uint32_t *wo
         if (n_c > n) throw Fail(LPA_ERR_ARG, "n_c > n");
         if (n_c >= (1ull << 32) - 1) throw Fail(LPA_ERR_RANGE, "too many compressed vertices");
         HIPCHK(hipSetDevice(e->device));
+        e->drop_graphs();
         e->have_graph = e->scheduled = e->have_labels = e->have_csr = false;
         if (offsets[0] != 0) throw Fail(LPA_ERR_FORMAT, "offsets[0] != 0");
         for (uint64_t i = 0; i < n_c; i++)
@@ -607,6 +623,7 @@ static void build_heavy(lpa_engine *e) {
 // positions first, then small positions).
 static void build_layout(lpa_engine *e, const std::vector<uint8_t> &kind_c, const std::vector<uint8_t> &kind_s,
                          const lpa_sched_params &p) {
+    e->drop_graphs();
     e->kind_of.assign(e->n, LPA_BIN_SKIP);
     for (uint64_t i = 0; i < e->n_c; i++) e->kind_of[e->h_vertices[i]] = kind_c[i];
     for (uint64_t j = 0; j < e->n_s; j++) e->kind_of[e->h_vertices[e->n_c + j]] = kind_s[j];
@@ -814,6 +831,7 @@ int lpa_labels_init(lpa_engine *e, const uint32_t *labels) {
         if (!e) throw Fail(LPA_ERR_ARG, "null argument");
         if (!e->have_graph) throw Fail(LPA_ERR_STATE, "no graph uploaded");
         HIPCHK(hipSetDevice(e->device));
+        e->drop_graphs();
         const uint64_t tot = e->n * e->K;
         std::vector<uint32_t> h(tot);
         if (labels) {
@@ -859,6 +877,44 @@ static void require_ready(lpa_engine *e) {
     if (!e->have_labels) throw Fail(LPA_ERR_STATE, "labels not initialised");
 }
 
+static void cglp_step(lpa_engine *e);
+
+// Enqueues one step (method 0 = fused, 1 = two-kernel) on the engine
+// stream: eagerly, or by replaying the step captured as a HIP graph.
+static void enqueue_step(lpa_engine *e, int method) {
+    if (method == 1 && !e->have_csr) {
+        // the first decode allocates the CSR: never inside a capture
+        cglp_step(e);
+        return;
+    }
+    if (!e->graph_mode) {
+        if (method == 0) e->step(false, nullptr);
+        else cglp_step(e);
+        return;
+    }
+    hipGraphExec_t &ge = e->graphs[method][e->cur];
+    if (!ge) {
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            if (method == 0) e->step(false, nullptr);
+            else cglp_step(e);
+        } catch (...) {
+            (void)hipStreamEndCapture(e->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HIPCHK(hipStreamEndCapture(e->stream, &g));
+        hipError_t err = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (err != hipSuccess) {
+            ge = nullptr;
+            throw Fail(LPA_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(err));
+        }
+    }
+    HIPCHK(hipGraphLaunch(ge, e->stream));
+}
+
 static void finish_step(lpa_engine *e, uint64_t *changed) {
     HIPCHK(hipMemcpyAsync(e->h_counter, e->counter.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, e->stream));
     e->sync();
@@ -869,8 +925,16 @@ static void finish_step(lpa_engine *e, uint64_t *changed) {
 int lpa_nova_iterate(lpa_engine *e, uint64_t *changed) {
     return guard([&] {
         require_ready(e);
-        e->step(false, nullptr);
+        enqueue_step(e, 0);
         finish_step(e, changed);
+    });
+}
+
+int lpa_set_graph_mode(lpa_engine *e, int on) {
+    return guard([&] {
+        if (!e) throw Fail(LPA_ERR_ARG, "null argument");
+        e->graph_mode = on != 0;
+        if (!e->graph_mode) e->drop_graphs();
     });
 }
 
@@ -935,7 +999,7 @@ static void cglp_step(lpa_engine *e) {
 int lpa_cglp_iterate(lpa_engine *e, uint64_t *changed) {
     return guard([&] {
         require_ready(e);
-        cglp_step(e);
+        enqueue_step(e, 1);
         finish_step(e, changed);
     });
 }
@@ -1269,14 +1333,17 @@ int lpa_time_iterations(lpa_engine *e, int method, int iters, int use_graph, flo
         for (auto &x : ev) HIPCHK(hipEventCreate(&x));
         for (auto &x : kev) HIPCHK(hipEventCreate(&x));
         std::vector<double> kind_sum(LPA_NBIN_KINDS, 0.0);
+        const bool had_graph_mode = e->graph_mode;
         try {
             if (method == 1 && !e->have_csr) {
                 cglp_decode(e);
                 e->sync();
             }
+            e->graph_mode = use_graph != 0;
             for (int it = 0; it < iters; it++) {
                 HIPCHK(hipEventRecord(ev[0], e->stream));
-                if (method == 0) e->step(false, kev);
+                if (e->graph_mode) enqueue_step(e, method);
+                else if (method == 0) e->step(false, kev);
                 else cglp_step(e);
                 HIPCHK(hipEventRecord(ev[1], e->stream));
                 HIPCHK(hipMemcpyAsync(e->h_counter, e->counter.p, 8, hipMemcpyDeviceToHost, e->stream));
@@ -1284,7 +1351,7 @@ int lpa_time_iterations(lpa_engine *e, int method, int iters, int use_graph, flo
                 HIPCHK(hipEventElapsedTime(&ms[it], ev[0], ev[1]));
                 if (changed) changed[it] = *e->h_counter;
                 e->cur ^= 1;
-                if (method == 0) {
+                if (method == 0 && !e->graph_mode) {
                     bool seen[LPA_NBIN_KINDS] = {false};
                     for (const Bin &b : e->bins)
                         if (b.kind != LPA_BIN_SKIP && !seen[b.kind] && b.end > b.begin) {
@@ -1296,14 +1363,58 @@ int lpa_time_iterations(lpa_engine *e, int method, int iters, int use_graph, flo
                 }
             }
         } catch (...) {
+            e->graph_mode = had_graph_mode;
             for (auto &x : ev) (void)hipEventDestroy(x);
             for (auto &x : kev) (void)hipEventDestroy(x);
             throw;
         }
+        e->graph_mode = had_graph_mode;
         for (int k = 0; k < LPA_NBIN_KINDS; k++) e->kind_ms[k] = (float)(kind_sum[k] / iters);
         for (auto &x : ev) HIPCHK(hipEventDestroy(x));
         for (auto &x : kev) HIPCHK(hipEventDestroy(x));
         (void)use_graph;
+    });
+}
+
+int lpa_autotune(lpa_engine *e, int iters, lpa_sched_params *best, float *best_ms) {
+    return guard([&] {
+        require_ready(e);
+        if (iters <= 0 || !best) throw Fail(LPA_ERR_ARG, "bad arguments");
+        // the labels are restored after every candidate: tuning never changes results
+        const uint64_t tot = e->n * e->K;
+        std::vector<uint32_t> saved(tot);
+        HIPCHK(hipMemcpy(saved.data(), e->L[e->cur].p, tot * 4, hipMemcpyDeviceToHost));
+        const int saved_cur = e->cur;
+        const uint32_t wc = wave_cap(e);
+        float best_t = 1e30f;
+        lpa_sched_params bp = e->sched_used;
+        std::vector<float> ms(iters);
+        for (uint32_t lane : {8u, 16u, 32u}) {
+            for (uint32_t wave : {64u, 128u, 256u, 512u, 1024u}) {
+                if (wave > wc) continue;
+                for (uint32_t quarter : {64u, 128u}) {
+                    lpa_sched_params q{lane, wave, 0, 1};
+                    q.quarter_max = quarter;
+                    int rc = lpa_schedule_degree(e, &q);
+                    if (rc) throw Fail(rc, g_err);
+                    rc = lpa_time_iterations(e, 0, iters, 0, ms.data(), nullptr);
+                    if (rc) throw Fail(rc, g_err);
+                    e->cur = saved_cur;
+                    HIPCHK(hipMemcpy(e->L[e->cur].p, saved.data(), tot * 4, hipMemcpyHostToDevice));
+                    std::vector<float> sorted(ms);
+                    std::sort(sorted.begin(), sorted.end());
+                    const float med = sorted[sorted.size() / 2];
+                    if (med < best_t) {
+                        best_t = med;
+                        bp = q;
+                    }
+                }
+            }
+        }
+        int rc = lpa_schedule_degree(e, &bp);
+        if (rc) throw Fail(rc, g_err);
+        *best = bp;
+        if (best_ms) *best_ms = best_t;
     });
 }
 

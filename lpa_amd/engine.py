@@ -39,6 +39,8 @@ EXPORTS = {
     "lpa_cglp_iterate": (C.c_int, [_VP, C.POINTER(C.c_uint64)]),
     "lpa_cglp_csr_get": (C.c_int, [_VP, _VP, _VP, C.POINTER(C.c_uint64)]),
     "lpa_time_iterations": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, _VP, _VP]),
+    "lpa_set_graph_mode": (C.c_int, [_VP, C.c_int]),
+    "lpa_autotune": (C.c_int, [_VP, C.c_int, _VP, _VP]),
     "lpa_model_features": (C.c_int, [_VP, _VP]),
     "lpa_record_family": (C.c_int, [_VP, C.c_int, _VP, _VP]),
     "lpa_mlp_create": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(_VP)]),
@@ -55,7 +57,7 @@ EXPORTS = {
 
 class SchedParams(C.Structure):
     _fields_ = [("lane_max", C.c_uint32), ("wave_max", C.c_uint32),
-                ("block_max", C.c_uint32), ("relayout", C.c_uint32)]
+                ("block_max", C.c_uint32), ("relayout", C.c_uint32), ("quarter_max", C.c_uint32)]
 
 
 def library_path():
@@ -149,9 +151,19 @@ class Engine:
         _check(self._L.lpa_lp_config(self._h, K, int(semi), cms_thresh, cms_depth, cms_width, cms_seed))
         self.K = K
 
-    def schedule_degree(self, lane_max=32, wave_max=256, block_max=0):
-        p = SchedParams(lane_max, wave_max, block_max, 1)
+    def schedule_degree(self, lane_max=32, wave_max=256, block_max=0, quarter_max=128):
+        p = SchedParams(lane_max, wave_max, block_max, 1, quarter_max)
         _check(self._L.lpa_schedule_degree(self._h, C.byref(p)))
+
+    def autotune(self, iters=3):
+        """fastest degree schedule for this graph; labels are left unchanged"""
+        p = SchedParams()
+        ms = C.c_float(0)
+        _check(self._L.lpa_autotune(self._h, iters, C.byref(p), C.byref(ms)))
+        return dict(lane_max=p.lane_max, wave_max=p.wave_max, quarter_max=p.quarter_max), ms.value
+
+    def set_graph_mode(self, on=True):
+        _check(self._L.lpa_set_graph_mode(self._h, int(on)))
 
     def schedule_bins(self, bins):
         b = np.ascontiguousarray(bins, np.uint8)

@@ -89,3 +89,39 @@ def test_malformed_stream_rejected():
     e = L.Engine(0)
     with pytest.raises(L.LpaError):
         e.upload_arrays(cgr.n, cgr.n_c, words, cgr.nwords, cgr.offsets, cgr.vertices)
+
+
+def test_graph_replay_matches_eager():
+    """steps replayed from captured HIP graphs, with eager work and
+    synchronisations in between, match the eager engine every time"""
+    g = L.gen_rmat(14, 16, seed=31)
+    cgr = g.encode(0, 128)
+    eager = make_engine(cgr, cms_thresh=1000, cms_depth=4, cms_width=256, cms_seed=2)
+    graph = make_engine(cgr, cms_thresh=1000, cms_depth=4, cms_width=256, cms_seed=2)
+    eager.labels_init()
+    graph.labels_init()
+    graph.set_graph_mode(True)
+    for it in range(8):
+        a = eager.nova_iterate()
+        b = graph.nova_iterate()
+        assert a == b and np.array_equal(eager.labels(), graph.labels()), it
+        graph.degrees()          # unrelated eager work between replays
+    # the two-kernel method through graphs as well
+    eager.labels_init()
+    graph.labels_init()
+    for it in range(4):
+        assert eager.cglp_iterate() == graph.cglp_iterate()
+        assert np.array_equal(eager.labels(), graph.labels()), it
+
+
+def test_autotune_keeps_results():
+    g = L.gen_rmat(14, 16, seed=32)
+    cgr = g.encode(0, 128)
+    e = make_engine(cgr)
+    e.labels_init()
+    e.nova_iterate()
+    before = e.labels()
+    best, ms = e.autotune(iters=2)
+    assert np.array_equal(e.labels(), before) and ms > 0
+    print("autotuned", best, ms)
+    run_and_compare(cgr, 3, engine=e)
