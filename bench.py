@@ -1,19 +1,23 @@
 #!/usr/bin/env python3
-"""Benchmark: LP edges/s per iteration on a CGR-compressed graph (MI355X).
+"""Benchmark: LP edges/s per iteration on a CGR graph, learned vs degree-binned schedule.
 
-Workload (default, "rmat22"): R-MAT scale 22 (4,194,304 vertices), edge
-factor 16, a/b/c/d = 0.57/0.19/0.19/0.05, seed 1, symmetrised, deduplicated,
-self loops removed, BFS-relabelled from vertex 0; CGR threshold 0; labels
-start as the vertex id; count-min sketch (depth 4, width 1024, seed 0) for
-degree > 4096.  One step = one synchronous fused decode+LP iteration (NOVA)
-over every vertex, inputs resident in HBM.  The CGLP two-kernel method is
-timed on the same graph and reported beside it.
+Default workload "rmat22": R-MAT scale 22 (4,194,304 vertices), edge factor
+16, a/b/c/d = 0.57/0.19/0.19/0.05, seed 1, symmetrised, deduplicated, self
+loops removed, BFS-relabelled from vertex 0; CGR threshold 0; labels start as
+the vertex id; count-min sketch (depth 4, width 1024, seed 0) for degree > 4096.
 
-Prints one JSON line.  `value` = m * steps / elapsed (arcs per second for
-K = 1); the roofline object describes the dominant kernel family, timed
-with HIP events on the engine's stream; cpu_baseline times the CPU oracle's
-decode+LP step on the host cores over a bounded random sample of the same
-graph's vertices (about 20 s of CPU work).
+One step = one synchronous fused decode+LP iteration (NOVA) over every vertex
+and every label vector, inputs resident in HBM, timed with HIP events on the
+engine's stream.  Edges/s = K * m / t_step.  Reported:
+  value            learned-schedule fused step with K = 8 label vectors
+                   (the cost model is trained on work recorded on the
+                   1024x1024 lattice and on this graph)
+  degree_k1        degree-binned fused step, K = 1 (the plain kernel)
+  degree_k8        degree-binned fused step, K = 8 (isolates the schedule's gain)
+  cglp_k1          the two-kernel method, K = 1
+The roofline object describes the dominant kernel family of the headline run;
+cpu_baseline times the CPU oracle's decode+LP step on the host cores over a
+bounded random sample of the same graph's vertices.
 """
 import argparse
 import json
@@ -29,10 +33,14 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
-    "rmat22": dict(gen="rmat", scale=22, ef=16, seed=1, T=0, K=1,
+    "rmat22": dict(gen="rmat", scale=22, ef=16, seed=1, T=0, K=8,
                    cms=dict(cms_thresh=4096, cms_depth=4, cms_width=1024, cms_seed=0)),
     "lattice1024": dict(gen="lattice", side=1024, seed=0, T=0, K=1, cms=dict()),
 }
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def build_graph(w):
@@ -44,12 +52,36 @@ def build_graph(w):
     return g, g.encode(w["T"], 128)
 
 
-def alg_bytes(vertices, arcs, bits, csr_arcs, K):
-    """algorithmic HBM bytes of one pass: compressed stream once, 8 B offset
-    per compressed vertex, per label vector 4 B gather per arc plus 4 B read
-    and 4 B write per vertex; small-vertex CSR: 4 B per arc + 8 B offset"""
-    n_c = vertices  # compressed or CSR vertices of this family
-    return bits / 8.0 + 8.0 * n_c + K * (4.0 * arcs + 8.0 * vertices) + 4.0 * csr_arcs
+def alg_bytes(st, K):
+    """algorithmic HBM bytes of one pass over a kernel family's vertices:
+    stream once, 8 B offset per vertex, per label vector a 4 B gather per
+    arc plus 4 B read and 4 B write per vertex; small-CSR arcs 4 B"""
+    return st["bits"] / 8.0 + 8.0 * st["vertices"] + K * (4.0 * st["arcs"] + 8.0 * st["vertices"]) \
+        + 4.0 * st["csr_arcs"]
+
+
+def timed(e, steps, warmup, method="nova"):
+    e.labels_init()
+    for _ in range(warmup):
+        e.nova_iterate() if method == "nova" else e.cglp_iterate()
+    ms, changed = e.time_iterations(steps, method)
+    return ms, changed, e.kernel_times()
+
+
+def train_model(L, cgr_main, w):
+    """cost model trained on work recorded on the lattice and on this graph"""
+    samples = []
+    lat = L.gen_lattice(1024, 0.10, 0.005, seed=0).encode(0, 128)
+    for cgr, cms in ((lat, {}), (cgr_main, w["cms"])):
+        e = L.Engine(0)
+        e.upload(cgr)
+        e.config(K=1, **cms)
+        e.schedule_degree()
+        e.labels_init()
+        e.nova_iterate()
+        samples.append(L.schedule.record_samples(e, max_samples=200000))
+        e.close()
+    return L.schedule.train_cost_model(samples, epochs=40, batch=2048, lr=2e-3)
 
 
 def main():
@@ -58,8 +90,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="rmat22", choices=sorted(WORKLOADS))
+    ap.add_argument("--K", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-cglp", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus})",
@@ -67,33 +101,51 @@ def main():
         sys.exit(2)
 
     import lpa_amd as L
-    w = WORKLOADS[args.workload]
+    w = dict(WORKLOADS[args.workload])
+    K = args.K or w["K"]
     t0 = time.time()
     g, cgr = build_graph(w)
     m = int(g.m)
-    print(f"[bench] graph n={g.n} m={m} bits={cgr.nbits} built in {time.time() - t0:.1f}s",
-          file=sys.stderr, flush=True)
+    log(f"graph n={g.n} m={m} bits={cgr.nbits} built in {time.time() - t0:.1f}s")
+
     e = L.Engine(0)
     e.upload(cgr)
-    e.config(K=w["K"], **w["cms"])
+    res = {}
+    # degree-binned plain kernel, K = 1
+    e.config(K=1, **w["cms"])
     e.schedule_degree()
+    ms, ch1, kt1 = timed(e, args.steps, args.warmup)
+    res["degree_k1"] = dict(edges_per_s=m / (float(np.mean(ms)) / 1e3), ms_per_step=float(np.mean(ms)),
+                            kernel_ms=kt1, changed=[int(x) for x in ch1])
+    log(f"degree K=1: {np.mean(ms):.3f} ms/step")
+    if not args.no_cglp:
+        ms2, _, _ = timed(e, max(2, args.steps // 2), 1, "cglp")
+        res["cglp_k1"] = dict(edges_per_s=m / (float(np.mean(ms2)) / 1e3), ms_per_step=float(np.mean(ms2)))
+        log(f"cglp K=1: {np.mean(ms2):.3f} ms/step")
+    # degree-binned, K vectors
+    e.config(K=K, **w["cms"])
+    e.schedule_degree()
+    msk, chk, ktk = timed(e, args.steps, args.warmup)
+    res[f"degree_k{K}"] = dict(edges_per_s=K * m / (float(np.mean(msk)) / 1e3), ms_per_step=float(np.mean(msk)))
+    log(f"degree K={K}: {np.mean(msk):.3f} ms/step")
+    head_ms, head_kt, schedule = msk, ktk, "degree"
+    if not args.no_learned:
+        model, losses = train_model(L, cgr, w)
+        e.config(K=K, **w["cms"])
+        e.schedule_learned(model)
+        msl, chl, ktl = timed(e, args.steps, args.warmup)
+        res[f"learned_k{K}"] = dict(edges_per_s=K * m / (float(np.mean(msl)) / 1e3),
+                                    ms_per_step=float(np.mean(msl)), model_loss=[float(losses[0]), float(losses[-1])],
+                                    bins=e.counts())
+        log(f"learned K={K}: {np.mean(msl):.3f} ms/step")
+        head_ms, head_kt, schedule = msl, ktl, "learned"
     stats = e.stats()
-    e.labels_init()
-    for _ in range(args.warmup):
-        e.nova_iterate()
-    t = time.perf_counter()
-    ms, changed = e.time_iterations(args.steps, "nova")
-    wall = time.perf_counter() - t
-    kt = e.kernel_times()
-    ms_per_step = float(np.mean(ms))
-    value = m * w["K"] * args.steps / (float(np.sum(ms)) / 1e3)
-
-    # dominant kernel family and its algorithmic bytes per step
-    dom = max((k for k in kt if stats[k]["vertices"]), key=lambda k: kt[k])
-    s = stats[dom]
-    dom_bytes = alg_bytes(s["vertices"], s["arcs"], s["bits"], s["csr_arcs"], w["K"])
-    achieved = dom_bytes / (kt[dom] / 1e3) / 1e9
-    it_bytes = (cgr.nbits / 8 + 8 * (cgr.n_c + 1) + w["K"] * (4 * m + 8 * g.n))
+    ms_per_step = float(np.mean(head_ms))
+    value = K * m * args.steps / (float(np.sum(head_ms)) / 1e3)
+    dom = max((k for k in head_kt if stats[k]["vertices"]), key=lambda k: head_kt[k])
+    dom_bytes = alg_bytes(stats[dom], K)
+    achieved = dom_bytes / (head_kt[dom] / 1e3) / 1e9
+    it_bytes = cgr.nbits / 8 + 8 * (cgr.n_c + 1) + K * (4 * m + 8 * g.n)
     line = {
         "metric": "LP edges/sec per iteration on CGR graph; learned vs degree-binned schedule",
         "value": value,
@@ -107,40 +159,35 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic",
-        "config": {"workload": args.workload, "n": int(g.n), "m": m, "cgr_bits": int(cgr.nbits),
-                   "K": w["K"], "method": "nova-fused", "schedule": "degree", **w["cms"]},
+        "config": {"workload": args.workload, "n": int(g.n), "m": m, "cgr_bits": int(cgr.nbits), "K": K,
+                   "method": "nova-fused", "schedule": schedule, **w["cms"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": dom, "kernel_ms_per_step": kt[dom], "alg_bytes_per_step": dom_bytes},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                     "kernel_ms_per_step": head_kt[dom], "alg_bytes_per_step": dom_bytes},
         "iteration_roofline": {"alg_bytes": it_bytes, "achieved_GBs": it_bytes / (ms_per_step / 1e3) / 1e9,
                                "frac": it_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS},
-        "kernel_ms_per_step": kt,
+        "kernel_ms_per_step": head_kt,
         "bins": {k: v for k, v in stats.items() if v["vertices"]},
-        "changed": [int(x) for x in changed],
         "device": e.info(),
-        "wall_s": wall,
+        **res,
     }
-    if not args.no_cglp:
-        e.labels_init()
-        ms2, _ = e.time_iterations(max(2, args.steps // 2), "cglp")
-        line["cglp_ms_per_step"] = float(np.mean(ms2))
-        line["cglp_edges_per_s"] = m * w["K"] / (float(np.mean(ms2)) / 1e3)
-        line["nova_over_cglp"] = line["cglp_ms_per_step"] / ms_per_step
+    line["speedup_vs_degree_k1"] = value / res["degree_k1"]["edges_per_s"]
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline_run(cgr, g.degrees(), w)
     print(json.dumps(line))
 
 
 def cpu_baseline_run(cgr, deg, w, budget_s=20.0):
-    """times the CPU oracle's decode+LP step on the host cores over a
-    bounded prefix of stream positions (whole iterations when they fit)"""
+    """times the CPU oracle's decode+LP step (K = 1) on the host cores over
+    a bounded random sample of stream positions (whole iterations when they
+    fit the budget)"""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
     cores = min(len(os.sched_getaffinity(0)), 16)
     view = O.CgrView(np.frombuffer(cgr.stream_bytes(), np.uint8), cgr.nbits, cgr.offsets.copy(),
                      cgr.vertices.copy(), cgr.small_offsets.copy(), cgr.small_nbrs.copy())
-    c = O.cfg(K=w["K"], nthreads=cores, **w["cms"])
-    lab = np.repeat(np.arange(cgr.n, dtype=np.uint32), w["K"])
+    c = O.cfg(K=1, nthreads=cores, **w["cms"])
+    lab = np.arange(cgr.n, dtype=np.uint32)
     out = lab.copy()
     rng = np.random.default_rng(0)
     pos_all = rng.permutation(cgr.n).astype(np.uint64)
@@ -148,13 +195,24 @@ def cpu_baseline_run(cgr, deg, w, budget_s=20.0):
     t = time.perf_counter()
     O.step_cgr_subset(view, lab, pos_all[:n_probe], out, c)
     probe_s = time.perf_counter() - t
+    full_s = probe_s * cgr.n / n_probe
+    if full_s * 3 <= budget_s:
+        reps = min(10, max(1, int(budget_s / max(full_s, 1e-3))))
+        t = time.perf_counter()
+        for _ in range(reps):
+            O.step_cgr(view, lab, c)
+        el = (time.perf_counter() - t) / reps
+        arcs = int(deg.sum())
+        return {"value": arcs / el, "unit": "edges/s", "cores": cores, "kind": "port",
+                "sample": f"{reps} full synchronous oracle steps over all {cgr.n} vertices ({arcs} arcs), "
+                          f"{el:.2f} s each"}
     n_take = int(min(cgr.n, max(n_probe, n_probe * budget_s / max(probe_s, 1e-3))))
     sample = np.sort(pos_all[:n_take])
     t = time.perf_counter()
     O.step_cgr_subset(view, lab, sample, out, c)
     el = time.perf_counter() - t
     arcs = int(deg[cgr.vertices[sample.astype(np.int64)]].sum())
-    return {"value": arcs * w["K"] / el, "unit": "edges/s", "cores": cores, "kind": "port",
+    return {"value": arcs / el, "unit": "edges/s", "cores": cores, "kind": "port",
             "sample": f"oracle decode+LP step over {n_take} of {cgr.n} random stream positions "
                       f"({arcs} arcs), {el:.1f} s"}
 
