@@ -33,10 +33,33 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
+    # R-MAT scale 22, ef 16, seed 1; CMS 4x1024 for deg > 4096 (default)
     "rmat22": dict(gen="rmat", scale=22, ef=16, seed=1, T=0, K=8,
                    cms=dict(cms_thresh=4096, cms_depth=4, cms_width=1024, cms_seed=0)),
+    # 1024x1024 lattice, 10% edges dropped, 0.5% shortcuts, seed 0; exact
     "lattice1024": dict(gen="lattice", side=1024, seed=0, T=0, K=1, cms=dict()),
+    # copying model: 50M vertices, mean out-degree 30, power law 2.1 capped at
+    # 100k, 70% copied within +-1000, seed 2, directed; T = 32; CMS for deg > 8192
+    "copy50m": dict(gen="copying", n=50_000_000, seed=2, T=32, K=1,
+                    cms=dict(cms_thresh=8192, cms_depth=4, cms_width=2048, cms_seed=0)),
+    # R-MAT scale 24, ef 16, seed 3; K = 64 semi-supervised vectors (1% seeds
+    # from 100 classes, clamped); exact
+    "rmat24k64": dict(gen="rmat", scale=24, ef=16, seed=3, T=0, K=64, semi=True, cms=dict()),
+    # R-MAT scale 27, ef 32, seed 4; K = 8; CMS 4x2048 for deg > 8192
+    "rmat27": dict(gen="rmat", scale=27, ef=32, seed=4, T=0, K=8,
+                   cms=dict(cms_thresh=8192, cms_depth=4, cms_width=2048, cms_seed=0)),
 }
+
+
+def semi_labels(n, K, seed, classes=100, frac=0.01):
+    """K label vectors: in each, frac of the vertices seeded uniformly with
+    one of `classes` labels, the rest unlabelled (0xFFFFFFFF)"""
+    rng = np.random.default_rng(seed)
+    lab = np.full((n, K), 0xFFFFFFFF, np.uint32)
+    for k in range(K):
+        idx = rng.choice(n, int(frac * n), replace=False)
+        lab[idx, k] = rng.integers(0, classes, len(idx), dtype=np.uint32)
+    return lab.reshape(-1)
 
 
 def log(msg):
@@ -47,6 +70,8 @@ def build_graph(w):
     import lpa_amd as L
     if w["gen"] == "rmat":
         g = L.gen_rmat(w["scale"], w["ef"], 0.57, 0.19, 0.19, seed=w["seed"], bfs=True)
+    elif w["gen"] == "copying":
+        g = L.gen_copying(w["n"], 30.0, 2.1, 100000, 0.7, 1000, seed=w["seed"])
     else:
         g = L.gen_lattice(w["side"], 0.10, 0.005, seed=w["seed"])
     return g, g.encode(w["T"], 128)
@@ -60,8 +85,8 @@ def alg_bytes(st, K):
         + 4.0 * st["csr_arcs"]
 
 
-def timed(e, steps, warmup, method="nova"):
-    e.labels_init()
+def timed(e, steps, warmup, method="nova", labels=None):
+    e.labels_init(labels)
     for _ in range(warmup):
         e.nova_iterate() if method == "nova" else e.cglp_iterate()
     ms, changed = e.time_iterations(steps, method)
@@ -72,7 +97,11 @@ def train_model(L, cgr_main, w):
     """cost model trained on work recorded on the lattice and on this graph"""
     samples = []
     lat = L.gen_lattice(1024, 0.10, 0.005, seed=0).encode(0, 128)
-    for cgr, cms in ((lat, {}), (cgr_main, w["cms"])):
+    r22 = cgr_main
+    if cgr_main.n > (1 << 22):
+        # the model learns from the lattice and R-MAT scale 22 only
+        r22 = L.gen_rmat(22, 16, 0.57, 0.19, 0.19, seed=1, bfs=True).encode(0, 128)
+    for cgr, cms in ((lat, {}), (r22, WORKLOADS["rmat22"]["cms"])):
         e = L.Engine(0)
         e.upload(cgr)
         e.config(K=1, **cms)
@@ -111,29 +140,32 @@ def main():
     e = L.Engine(0)
     e.upload(cgr)
     res = {}
+    semi = bool(w.get("semi"))
+    lab1 = semi_labels(g.n, 1, w["seed"]) if semi else None
+    labK = semi_labels(g.n, K, w["seed"]) if semi else None
     # degree-binned plain kernel, K = 1
-    e.config(K=1, **w["cms"])
+    e.config(K=1, semi=semi, **w["cms"])
     e.schedule_degree()
-    ms, ch1, kt1 = timed(e, args.steps, args.warmup)
+    ms, ch1, kt1 = timed(e, args.steps, args.warmup, labels=lab1)
     res["degree_k1"] = dict(edges_per_s=m / (float(np.mean(ms)) / 1e3), ms_per_step=float(np.mean(ms)),
                             kernel_ms=kt1, changed=[int(x) for x in ch1])
     log(f"degree K=1: {np.mean(ms):.3f} ms/step")
     if not args.no_cglp:
-        ms2, _, _ = timed(e, max(2, args.steps // 2), 1, "cglp")
+        ms2, _, _ = timed(e, max(2, args.steps // 2), 1, "cglp", labels=lab1)
         res["cglp_k1"] = dict(edges_per_s=m / (float(np.mean(ms2)) / 1e3), ms_per_step=float(np.mean(ms2)))
         log(f"cglp K=1: {np.mean(ms2):.3f} ms/step")
     # degree-binned, K vectors
-    e.config(K=K, **w["cms"])
+    e.config(K=K, semi=semi, **w["cms"])
     e.schedule_degree()
-    msk, chk, ktk = timed(e, args.steps, args.warmup)
+    msk, chk, ktk = timed(e, args.steps, args.warmup, labels=labK)
     res[f"degree_k{K}"] = dict(edges_per_s=K * m / (float(np.mean(msk)) / 1e3), ms_per_step=float(np.mean(msk)))
     log(f"degree K={K}: {np.mean(msk):.3f} ms/step")
     head_ms, head_kt, schedule = msk, ktk, "degree"
     if not args.no_learned:
         model, losses = train_model(L, cgr, w)
-        e.config(K=K, **w["cms"])
+        e.config(K=K, semi=semi, **w["cms"])
         e.schedule_learned(model)
-        msl, chl, ktl = timed(e, args.steps, args.warmup)
+        msl, chl, ktl = timed(e, args.steps, args.warmup, labels=labK)
         res[f"learned_k{K}"] = dict(edges_per_s=K * m / (float(np.mean(msl)) / 1e3),
                                     ms_per_step=float(np.mean(msl)), model_loss=[float(losses[0]), float(losses[-1])],
                                     bins=e.counts())
@@ -160,7 +192,8 @@ def main():
         "dtype": "u32",
         "data": "synthetic",
         "config": {"workload": args.workload, "n": int(g.n), "m": m, "cgr_bits": int(cgr.nbits), "K": K,
-                   "method": "nova-fused", "schedule": schedule, **w["cms"]},
+                   "semi_supervised": semi, "threshold": w["T"], "method": "nova-fused", "schedule": schedule,
+                   **w["cms"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
                      "kernel_ms_per_step": head_kt[dom], "alg_bytes_per_step": dom_bytes},
@@ -186,8 +219,8 @@ def cpu_baseline_run(cgr, deg, w, budget_s=20.0):
     cores = min(len(os.sched_getaffinity(0)), 16)
     view = O.CgrView(np.frombuffer(cgr.stream_bytes(), np.uint8), cgr.nbits, cgr.offsets.copy(),
                      cgr.vertices.copy(), cgr.small_offsets.copy(), cgr.small_nbrs.copy())
-    c = O.cfg(K=1, nthreads=cores, **w["cms"])
-    lab = np.arange(cgr.n, dtype=np.uint32)
+    c = O.cfg(K=1, nthreads=cores, semi=bool(w.get("semi")), **w["cms"])
+    lab = semi_labels(cgr.n, 1, w["seed"]) if w.get("semi") else np.arange(cgr.n, dtype=np.uint32)
     out = lab.copy()
     rng = np.random.default_rng(0)
     pos_all = rng.permutation(cgr.n).astype(np.uint64)
