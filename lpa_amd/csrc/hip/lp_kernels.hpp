@@ -272,25 +272,29 @@ __device__ __forceinline__ uint32_t group_decode(const StepArgs &a, uint32_t p, 
         sh->qn = 0;
     }
     group_sync<G>();
-    uint64_t base;
-    const uint32_t ni = block_header(a.words, a.offs[p], &base);
+    // every lane reads the interval header; the lane of segment j seeks it
+    BitReader r;
+    r.seek(a.words, a.offs[p]);
+    const uint32_t ni = r.gamma() + 1;
+    const uint64_t base = r.pos;
+    auto emit_run = [&](uint32_t l, uint32_t len) {
+        const uint32_t o = atomicAdd(&sh->cnt, len);
+        if (len >= LONG_RUN) {
+            const uint32_t q = atomicAdd(&sh->qn, 1u);
+            if (q < (uint32_t)GroupShared<G>::QCAP) {
+                sh->qleft[q] = l;
+                sh->qlen[q] = len;
+                sh->qpos[q] = o;
+                return;
+            }
+        }
+        for (uint32_t x = 0; x < len; x++) list[o + x] = l + x;
+    };
     uint64_t my_end = 0;
     for (uint32_t j = gl; j < ni; j += G) {
-        uint64_t e = decode_interval_segment(a.words, base + (uint64_t)SEG_BITS * j, v,
-                                             [&](uint32_t l, uint32_t len) {
-                                                 const uint32_t o = atomicAdd(&sh->cnt, len);
-                                                 if (len >= LONG_RUN) {
-                                                     const uint32_t q = atomicAdd(&sh->qn, 1u);
-                                                     if (q < (uint32_t)GroupShared<G>::QCAP) {
-                                                         sh->qleft[q] = l;
-                                                         sh->qlen[q] = len;
-                                                         sh->qpos[q] = o;
-                                                         return;
-                                                     }
-                                                 }
-                                                 for (uint32_t x = 0; x < len; x++) list[o + x] = l + x;
-                                             });
-        if (j == ni - 1) my_end = e;
+        r.move_to(base + (uint64_t)SEG_BITS * j);
+        read_interval_segment(r, v, emit_run);
+        if (j == ni - 1) my_end = r.pos;
     }
     uint64_t rs = 0;
     if (G <= 64) {
@@ -301,21 +305,14 @@ __device__ __forceinline__ uint32_t group_decode(const StepArgs &a, uint32_t p, 
         group_sync<G>();
         rs = sh->res_start;
     }
-    const uint32_t nr = block_header(a.words, rs, &base);
-    // the reader of segment j+G is loaded before segment j is decoded, so a
-    // lane with several segments overlaps the next load with decoding
-    BitReader r;
-    if (gl < nr) r.seek(a.words, base + (uint64_t)SEG_BITS * gl);
+    r.seek(a.words, rs);
+    const uint32_t nr = r.gamma() + 1;
+    const uint64_t rbase = r.pos;
     for (uint32_t j = gl; j < nr; j += G) {
-        BitReader cur = r;
-        if (j + G < nr) r.seek(a.words, base + (uint64_t)SEG_BITS * (j + G));
-        const uint32_t c = cur.gamma();
-        uint32_t o = atomicAdd(&sh->cnt, c);
-        uint32_t u = 0;
-        for (uint32_t t = 0; t < c; t++) {
-            u = (t == 0) ? from_nat(v, cur.zeta()) : u + cur.zeta() + 1;
-            list[o++] = u;
-        }
+        r.move_to(rbase + (uint64_t)SEG_BITS * j);
+        BitReader peek = r;   // the segment's count claims its list slots
+        uint32_t o = atomicAdd(&sh->cnt, peek.gamma());
+        read_residual_segment(r, v, [&](uint32_t u) { list[o++] = u; });
     }
     group_sync<G>();
     const uint32_t nq = sh->qn < (uint32_t)GroupShared<G>::QCAP ? sh->qn : (uint32_t)GroupShared<G>::QCAP;
