@@ -158,25 +158,82 @@ uint32_t lpah_csr_max_degree(const lpah_csr *g) {
 }
 void lpah_csr_free(lpah_csr *g) { delete g; }
 
+// BFS order from `root` with neighbours taken in ascending order, computed
+// level by level in parallel: a vertex discovered at level L+1 belongs to
+// the frontier vertex of smallest position that has it as a neighbour, and
+// the new level is ordered by (that position, vertex id) -- exactly the
+// order a sequential queue-based BFS produces.
+static void bfs_order(const lpah_csr *g, uint32_t root, std::vector<uint32_t> &order, std::vector<uint32_t> &nid) {
+    const uint64_t n = g->n;
+    const uint32_t NONE = 0xFFFFFFFFu;
+    nid.assign(n, NONE);
+    std::vector<uint32_t> claim(n, NONE);
+    order.clear();
+    order.reserve(n);
+    order.push_back(root);
+    nid[root] = 0;
+    uint64_t lo = 0, hi = 1;   // current frontier = order[lo, hi)
+    std::vector<uint64_t> cnt;
+    while (lo < hi) {
+        const uint64_t fsz = hi - lo;
+        // pass 1: every undiscovered neighbour remembers its first parent
+#pragma omp parallel for schedule(dynamic, 256)
+        for (uint64_t i = 0; i < fsz; i++) {
+            const uint32_t u = order[lo + i];
+            for (uint64_t e = g->row[u]; e < g->row[u + 1]; e++) {
+                const uint32_t w = g->col[e];
+                if (nid[w] != NONE) continue;
+                uint32_t cur = __atomic_load_n(&claim[w], __ATOMIC_RELAXED);
+                while ((uint32_t)i < cur &&
+                       !__atomic_compare_exchange_n(&claim[w], &cur, (uint32_t)i, true, __ATOMIC_RELAXED,
+                                                    __ATOMIC_RELAXED)) {
+                }
+            }
+        }
+        // pass 2: each parent counts its children, in adjacency order
+        cnt.assign(fsz + 1, 0);
+#pragma omp parallel for schedule(dynamic, 256)
+        for (uint64_t i = 0; i < fsz; i++) {
+            const uint32_t u = order[lo + i];
+            uint64_t c = 0;
+            for (uint64_t e = g->row[u]; e < g->row[u + 1]; e++) {
+                const uint32_t w = g->col[e];
+                if (nid[w] == NONE && claim[w] == (uint32_t)i) c++;
+            }
+            cnt[i + 1] = c;
+        }
+        for (uint64_t i = 0; i < fsz; i++) cnt[i + 1] += cnt[i];
+        order.resize(hi + cnt[fsz]);
+        // pass 3: children take consecutive positions
+#pragma omp parallel for schedule(dynamic, 256)
+        for (uint64_t i = 0; i < fsz; i++) {
+            const uint32_t u = order[lo + i];
+            uint64_t o = hi + cnt[i];
+            for (uint64_t e = g->row[u]; e < g->row[u + 1]; e++) {
+                const uint32_t w = g->col[e];
+                if (nid[w] == NONE && claim[w] == (uint32_t)i) order[o++] = w;
+            }
+        }
+        const uint64_t nhi = order.size();
+#pragma omp parallel for schedule(static)
+        for (uint64_t q = hi; q < nhi; q++) nid[order[q]] = (uint32_t)q;
+        lo = hi;
+        hi = nhi;
+    }
+    for (uint64_t v = 0; v < n; v++)
+        if (nid[v] == NONE) {
+            nid[v] = (uint32_t)order.size();
+            order.push_back((uint32_t)v);
+        }
+}
+
 int lpah_csr_bfs_relabel(const lpah_csr *g, uint32_t root, uint32_t *perm_out, lpah_csr **out) {
     return guard([&] {
         const uint64_t n = g->n;
         if (n == 0) { *out = new lpah_csr(*g); return; }
         if (root >= n) throw Error(E_RANGE, "root >= n");
-        const uint32_t NONE = 0xFFFFFFFFu;
-        std::vector<uint32_t> nid(n, NONE), order;
-        order.reserve(n);
-        nid[root] = 0;
-        order.push_back(root);
-        for (uint64_t h = 0; h < order.size(); h++) {
-            uint32_t u = order[h];
-            for (uint64_t e = g->row[u]; e < g->row[u + 1]; e++) {
-                uint32_t w = g->col[e];
-                if (nid[w] == NONE) { nid[w] = (uint32_t)order.size(); order.push_back(w); }
-            }
-        }
-        for (uint64_t v = 0; v < n; v++)
-            if (nid[v] == NONE) { nid[v] = (uint32_t)order.size(); order.push_back((uint32_t)v); }
+        std::vector<uint32_t> nid, order;
+        bfs_order(g, root, order, nid);
         auto *r = new lpah_csr;
         r->n = n;
         r->row.assign(n + 1, 0);

@@ -141,3 +141,27 @@ def test_loader_errors(tmp_path):
 def test_duplicate_and_self_loop_edges_are_removed():
     c = L.Csr.from_edges(4, [0, 0, 1, 2, 2], [1, 1, 1, 3, 3], undirected=True)
     assert list(c.row) == [0, 1, 2, 3, 4] and list(c.col) == [1, 0, 3, 2]
+
+
+def test_parallel_bfs_equals_sequential_bfs():
+    import collections
+    for seed in range(4):
+        g = L.gen_rmat(11, 8, seed=40 + seed, bfs=False)
+        r, perm = g.bfs_relabel(0)
+        row, col = g.row.astype(np.int64), g.col
+        seen = np.full(g.n, -1)
+        seen[0] = 0
+        q = collections.deque([0])
+        order = [0]
+        while q:
+            u = q.popleft()
+            for w in col[row[u]:row[u + 1]]:
+                if seen[w] < 0:
+                    seen[w] = len(order)
+                    order.append(int(w))
+                    q.append(int(w))
+        for v in range(g.n):
+            if seen[v] < 0:
+                seen[v] = len(order)
+                order.append(v)
+        assert np.array_equal(perm, seen)
