@@ -10,6 +10,8 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
+#include <cstdlib>
 #include <cstdio>
 #include <thread>
 
@@ -20,54 +22,120 @@ namespace lpah {
 static thread_local std::string g_err;
 void set_error(const std::string &m) { g_err = m; }
 
+void phase(const char *name) {
+    static const bool on = getenv("LPA_HOST_TIMING") != nullptr;
+    static auto last = std::chrono::steady_clock::now();
+    if (!on) return;
+    auto now = std::chrono::steady_clock::now();
+    fprintf(stderr, "[lpa host] %-28s %8.3f s\n", name, std::chrono::duration<double>(now - last).count());
+    last = now;
+}
+
 static void exclusive_scan(std::vector<uint64_t> &a) {
     uint64_t s = 0;
     for (auto &x : a) { uint64_t t = x; x = s; s += t; }
 }
 
+// Arcs are distributed without atomics: per-thread counts of arcs per
+// source bucket (a range of 2^shift source ids), a prefix sum over
+// (bucket, thread), a scatter of (src, dst) pairs into bucket order, then one
+// thread per bucket counts, places and sorts the rows of its id range in a
+// cache-sized window; rows are then sorted over all threads.  Atomic
+// counters would serialise every thread on the cache lines of R-MAT hubs.
 void build_csr(uint64_t n, std::vector<uint32_t> &src, std::vector<uint32_t> &dst,
                bool undirected, lpah_csr &out) {
     const uint64_t m = src.size();
     if (dst.size() != m) throw Error(E_ARG, "src/dst length mismatch");
-    std::vector<uint64_t> cnt(n + 1, 0);
-#pragma omp parallel for schedule(static)
-    for (uint64_t i = 0; i < m; i++) {
-        uint32_t u = src[i], v = dst[i];
-        if (u >= n || v >= n) continue;   // validated below
-        if (u == v) continue;
-        __atomic_fetch_add(&cnt[u], 1, __ATOMIC_RELAXED);
-        if (undirected) __atomic_fetch_add(&cnt[v], 1, __ATOMIC_RELAXED);
-    }
     for (uint64_t i = 0; i < m; i++)
         if (src[i] >= n || dst[i] >= n) throw Error(E_RANGE, "edge endpoint >= n");
-    exclusive_scan(cnt);              // cnt[v] = start of row v, cnt[n] = total
-    std::vector<uint64_t> cur(cnt.begin(), cnt.end() - 1);
-    std::vector<uint32_t> col0(cnt[n]);
-#pragma omp parallel for schedule(static)
-    for (uint64_t i = 0; i < m; i++) {
-        uint32_t u = src[i], v = dst[i];
-        if (u == v) continue;
-        col0[__atomic_fetch_add(&cur[u], 1, __ATOMIC_RELAXED)] = v;
-        if (undirected) col0[__atomic_fetch_add(&cur[v], 1, __ATOMIC_RELAXED)] = u;
+    uint32_t lg = 0;
+    while ((1ull << lg) < std::max<uint64_t>(n, 1)) lg++;
+    const uint32_t shift = lg > 10 ? lg - 10 : 0;
+    const uint64_t nb = ((std::max<uint64_t>(n, 1) - 1) >> shift) + 1;
+    const int T = omp_get_max_threads();
+    std::vector<uint64_t> hist((uint64_t)T * nb, 0);
+    const uint64_t chunk = (m + T - 1) / std::max(T, 1);
+#pragma omp parallel num_threads(T)
+    {
+        const int t = omp_get_thread_num();
+        uint64_t *h = &hist[(uint64_t)t * nb];
+        const uint64_t b0 = std::min(m, t * chunk), b1 = std::min(m, b0 + chunk);
+        for (uint64_t i = b0; i < b1; i++) {
+            const uint32_t u = src[i], v = dst[i];
+            if (u == v) continue;
+            h[u >> shift]++;
+            if (undirected) h[v >> shift]++;
+        }
+    }
+    // bucket-major prefix: bstart[b] = first slot of bucket b
+    phase("csr: bucket counts");
+    std::vector<uint64_t> bstart(nb + 1, 0);
+    uint64_t total = 0;
+    for (uint64_t b = 0; b < nb; b++) {
+        bstart[b] = total;
+        for (int t = 0; t < T; t++) {
+            uint64_t c = hist[(uint64_t)t * nb + b];
+            hist[(uint64_t)t * nb + b] = total;
+            total += c;
+        }
+    }
+    bstart[nb] = total;
+    std::vector<uint32_t> as(total), ad(total);
+#pragma omp parallel num_threads(T)
+    {
+        const int t = omp_get_thread_num();
+        uint64_t *h = &hist[(uint64_t)t * nb];
+        const uint64_t b0 = std::min(m, t * chunk), b1 = std::min(m, b0 + chunk);
+        for (uint64_t i = b0; i < b1; i++) {
+            const uint32_t u = src[i], v = dst[i];
+            if (u == v) continue;
+            uint64_t o = h[u >> shift]++;
+            as[o] = u;
+            ad[o] = v;
+            if (undirected) {
+                o = h[v >> shift]++;
+                as[o] = v;
+                ad[o] = u;
+            }
+        }
     }
     std::vector<uint32_t>().swap(src);
     std::vector<uint32_t>().swap(dst);
-    std::vector<uint64_t>().swap(cur);
+    std::vector<uint64_t>().swap(hist);
+    phase("csr: scatter to buckets");
+    // per bucket: degrees, then rows placed, sorted and deduplicated in place
     std::vector<uint64_t> deg(n + 1, 0);
-#pragma omp parallel for schedule(dynamic, 4096)
-    for (uint64_t v = 0; v < n; v++) {
-        auto b = col0.begin() + cnt[v], e = col0.begin() + cnt[v + 1];
-        std::sort(b, e);
-        deg[v] = (uint64_t)(std::unique(b, e) - b);
+    std::vector<uint32_t> col0(total);
+    std::vector<uint64_t> start(n + 1, 0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (uint64_t b = 0; b < nb; b++) {
+        const uint64_t v0 = b << shift, v1 = std::min<uint64_t>(n, (b + 1) << shift);
+        std::vector<uint64_t> cur(v1 - v0 + 1, 0);
+        for (uint64_t e = bstart[b]; e < bstart[b + 1]; e++) cur[as[e] - v0 + 1]++;
+        for (uint64_t k = 0; k < v1 - v0; k++) cur[k + 1] += cur[k];
+        for (uint64_t k = 0; k < v1 - v0; k++) start[v0 + k] = bstart[b] + cur[k];
+        for (uint64_t e = bstart[b]; e < bstart[b + 1]; e++) col0[bstart[b] + cur[as[e] - v0]++] = ad[e];
     }
+    start[n] = total;
+    phase("csr: place rows");
+    std::vector<uint32_t>().swap(as);
+    std::vector<uint32_t>().swap(ad);
+    // rows sorted over all threads (hub rows would serialise a bucket)
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (uint64_t v = 0; v < n; v++) {
+        auto it0 = col0.begin() + start[v], it1 = col0.begin() + start[v + 1];
+        std::sort(it0, it1);
+        deg[v] = (uint64_t)(std::unique(it0, it1) - it0);
+    }
+    phase("csr: sort rows");
     exclusive_scan(deg);
     out.n = n;
     out.col.assign(deg[n], 0);
 #pragma omp parallel for schedule(dynamic, 4096)
     for (uint64_t v = 0; v < n; v++)
-        std::copy(col0.begin() + cnt[v], col0.begin() + cnt[v] + (deg[v + 1] - deg[v]),
-                  out.col.begin() + deg[v]);
+        std::copy(col0.begin() + start[v], col0.begin() + start[v] + (deg[v + 1] - deg[v]), out.col.begin() + deg[v]);
     out.row.swap(deg);
+    phase("csr: compact");
 }
 
 }  // namespace lpah

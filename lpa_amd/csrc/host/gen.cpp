@@ -47,24 +47,35 @@ void gen_lattice(uint32_t side, double p_drop, double shortcut_frac, uint64_t se
     build_csr(n, s, d, true, out);
 }
 
+// Draw i takes its quadrant choices from 21-bit fields of the random words
+// ctr_rand(key, i * ceil(scale / 3) + j): level l uses field l % 3 of word
+// l / 3, compared with the thresholds a, a+b, a+b+c scaled to 2^21.
 void gen_rmat(uint32_t scale, uint32_t ef, double a, double b, double c, uint64_t seed,
               lpah_csr &out) {
     const uint64_t n = 1ull << scale, m = (uint64_t)ef << scale;
     const uint64_t key = stream_key(seed, 3);
-    const double ab = a + b, abc = a + b + c;
+    const uint32_t F = 1u << 21;
+    const uint32_t ta = (uint32_t)std::llround(a * F), tab = (uint32_t)std::llround((a + b) * F),
+                   tabc = (uint32_t)std::llround((a + b + c) * F);
+    const uint32_t words = (scale + 2) / 3;
+    phase("rmat: start");
     std::vector<uint32_t> s(m), d(m);
 #pragma omp parallel for schedule(static)
     for (uint64_t i = 0; i < m; i++) {
         uint32_t u = 0, v = 0;
-        for (uint32_t l = 0; l < scale; l++) {
-            double r = u01(ctr_rand(key, i * scale + l));
-            uint32_t bu = r >= ab, bv = (r >= a && r < ab) || r >= abc;
-            u = (u << 1) | bu;
-            v = (v << 1) | bv;
+        for (uint32_t w = 0; w < words; w++) {
+            const uint64_t r = ctr_rand(key, i * words + w);
+            for (uint32_t f = 0; f < 3 && 3 * w + f < scale; f++) {
+                const uint32_t x = (uint32_t)(r >> (21 * f)) & (F - 1);
+                const uint32_t bu = x >= tab, bv = (x >= ta && x < tab) || x >= tabc;
+                u = (u << 1) | bu;
+                v = (v << 1) | bv;
+            }
         }
         s[i] = u;
         d[i] = v;
     }
+    phase("rmat: draws");
     build_csr(n, s, d, true, out);
 }
 

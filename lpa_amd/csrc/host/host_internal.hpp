@@ -67,6 +67,9 @@ static inline uint64_t ctr_rand(uint64_t key, uint64_t i) {
 }
 static inline double u01(uint64_t r) { return (double)(r >> 11) * 0x1.0p-53; }
 
+// Phase timing of graph preparation on stderr when LPA_HOST_TIMING is set.
+void phase(const char *name);
+
 // Builds a CSR from arcs (src[i] -> dst[i]); removes self loops and
 // duplicates and sorts rows.  Consumes the arrays.
 void build_csr(uint64_t n, std::vector<uint32_t> &src, std::vector<uint32_t> &dst,
