@@ -133,6 +133,7 @@ struct lpa_engine {
     uint64_t n = 0, n_c = 0, n_s = 0, nbits = 0, m_c = 0, m_s = 0;
     DevBuf<uint64_t> words0, offs0;
     DevBuf<uint32_t> vid0;
+    DevBuf<uint4> meta0;                    // segment metadata per file-order compressed position
     std::vector<uint64_t> h_offs;           // file-order start offsets
     std::vector<uint32_t> h_vertices;       // file order
     std::vector<uint64_t> h_soffs;          // small CSR (file order)
@@ -144,6 +145,7 @@ struct lpa_engine {
     // scheduled device layout
     DevBuf<uint64_t> words, offs, srow;
     DevBuf<uint32_t> vid, snbr, svid;
+    DevBuf<uint4> meta;
     std::vector<Bin> bins;
     std::vector<uint32_t> order_c;          // device position -> file-order position
     std::vector<uint32_t> order_s;          // same for the small-vertex CSR
@@ -207,6 +209,7 @@ struct lpa_engine {
         a.words = words.p;
         a.offs = offs.p;
         a.vid = vid.p;
+        a.meta = meta.p;
         a.row = srow.p;
         a.nbr = snbr.p;
         a.cvid = svid.p;
@@ -488,12 +491,13 @@ int lpa_graph_upload(lpa_engine *e, uint64_t n, uint64_t n_c, const uint32_t *wo
         DevBuf<uint32_t> ddeg;
         ddeg.alloc(n_c);
         e->feat0.alloc(n_c * NFEAT);
+        e->meta0.alloc(n_c);
         e->counter.alloc(1);
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, e->stream, e->counter.p, (uint64_t)1, 0ull);
         if (n_c)
             hipLaunchKernelGGL(k_scan_records, dim3((unsigned)((n_c + 255) / 256)), dim3(256), 0, e->stream,
                                e->words0.p, e->offs0.p, e->vid0.p, n_c, (uint32_t)n, ddeg.p, e->feat0.p,
-                               e->counter.p);
+                               e->meta0.p, e->counter.p);
         HIPCHK(hipGetLastError());
         unsigned long long bad = 0;
         HIPCHK(hipMemcpyAsync(&bad, e->counter.p, sizeof bad, hipMemcpyDeviceToHost, e->stream));
@@ -678,12 +682,13 @@ static void build_layout(lpa_engine *e, const std::vector<uint8_t> &kind_c, cons
             HIPCHK(hipMemsetAsync(e->words.p, 0, n64 * 8, e->stream));
             e->offs.upload(noffs.data(), cnt + 1, e->stream);
             e->vid.alloc(std::max<uint64_t>(cnt, 1));
+            e->meta.alloc(std::max<uint64_t>(cnt, 1));
             DevBuf<uint32_t> perm;
             perm.upload(order.data(), cnt, e->stream);
             if (cnt)
                 hipLaunchKernelGGL(k_relayout, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, e->stream,
-                                   e->words0.p, e->offs0.p, e->vid0.p, perm.p, e->offs.p, e->words.p, e->vid.p,
-                                   cnt);
+                                   e->words0.p, e->offs0.p, e->vid0.p, e->meta0.p, perm.p, e->offs.p, e->words.p,
+                                   e->vid.p, e->meta.p, cnt);
             HIPCHK(hipGetLastError());
             e->sync();
         } else {

@@ -45,6 +45,7 @@ struct StepArgs {
     const uint64_t *__restrict__ words;
     const uint64_t *__restrict__ offs;     // start bit offset per position
     const uint32_t *__restrict__ vid;      // vertex id per position
+    const uint4 *__restrict__ meta;        // segment metadata per position (see k_scan_records)
     // CSR part (small vertices, or the CGLP decoded CSR)
     const uint64_t *__restrict__ row;      // per position: [row[p], row[p+1])
     const uint32_t *__restrict__ nbr;
@@ -272,11 +273,11 @@ __device__ __forceinline__ uint32_t group_decode(const StepArgs &a, uint32_t p, 
         sh->qn = 0;
     }
     group_sync<G>();
-    // every lane reads the interval header; the lane of segment j seeks it
-    BitReader r;
-    r.seek(a.words, a.offs[p]);
-    const uint32_t ni = r.gamma() + 1;
-    const uint64_t base = r.pos;
+    // segment j of the record (intervals first, then residuals) is at a
+    // position known from the metadata: every lane seeks its own segment
+    const uint64_t start = a.offs[p];
+    const uint4 m = a.meta[p];
+    const uint32_t ni = m.y, ns = m.y + m.w;
     auto emit_run = [&](uint32_t l, uint32_t len) {
         const uint32_t o = atomicAdd(&sh->cnt, len);
         if (len >= LONG_RUN) {
@@ -290,29 +291,17 @@ __device__ __forceinline__ uint32_t group_decode(const StepArgs &a, uint32_t p, 
         }
         for (uint32_t x = 0; x < len; x++) list[o + x] = l + x;
     };
-    uint64_t my_end = 0;
-    for (uint32_t j = gl; j < ni; j += G) {
-        r.move_to(base + (uint64_t)SEG_BITS * j);
-        read_interval_segment(r, v, emit_run);
-        if (j == ni - 1) my_end = r.pos;
-    }
-    uint64_t rs = 0;
-    if (G <= 64) {
-        const int src = (int)((__lane_id() & ~(uint32_t)(G - 1)) + ((ni - 1) & (G - 1)));
-        rs = __shfl(my_end, src, 64);
-    } else {
-        if (gl == ((ni - 1) & (G - 1))) sh->res_start = my_end;
-        group_sync<G>();
-        rs = sh->res_start;
-    }
-    r.seek(a.words, rs);
-    const uint32_t nr = r.gamma() + 1;
-    const uint64_t rbase = r.pos;
-    for (uint32_t j = gl; j < nr; j += G) {
-        r.move_to(rbase + (uint64_t)SEG_BITS * j);
-        BitReader peek = r;   // the segment's count claims its list slots
-        uint32_t o = atomicAdd(&sh->cnt, peek.gamma());
-        read_residual_segment(r, v, [&](uint32_t u) { list[o++] = u; });
+    for (uint32_t j = gl; j < ns; j += G) {
+        BitReader r;
+        if (j < ni) {
+            r.seek(a.words, start + m.x + (uint64_t)SEG_BITS * j);
+            read_interval_segment(r, v, emit_run);
+        } else {
+            r.seek(a.words, start + m.z + (uint64_t)SEG_BITS * (j - ni));
+            BitReader peek = r;   // the segment's count claims its list slots
+            uint32_t o = atomicAdd(&sh->cnt, peek.gamma());
+            read_residual_segment(r, v, [&](uint32_t u) { list[o++] = u; });
+        }
     }
     group_sync<G>();
     const uint32_t nq = sh->qn < (uint32_t)GroupShared<G>::QCAP ? sh->qn : (uint32_t)GroupShared<G>::QCAP;

@@ -44,13 +44,16 @@ struct SafeReader {
 // One lane per compressed position: decodes the record with every bound
 // checked (codes inside the record, padded segments inside 256 bits,
 // neighbour ids < n, record ends exactly at the next offset).  Writes the
-// degree and cost features per position; sets *err to 1 + position on the
-// first malformed record it sees (any one of them).
+// degree, cost features and segment metadata per position -- {bits from the
+// record start to the first interval segment, #interval segments, bits to
+// the first residual segment, #residual segments}, which lets a group seek
+// every segment at once -- and sets *err to 1 + position on the first
+// malformed record it sees (any one of them).
 __global__ void __launch_bounds__(256) k_scan_records(const uint64_t *__restrict__ W,
                                                       const uint64_t *__restrict__ offs,
                                                       const uint32_t *__restrict__ vid, uint64_t n_c,
                                                       uint32_t n, uint32_t *__restrict__ deg,
-                                                      float *__restrict__ feat,
+                                                      float *__restrict__ feat, uint4 *__restrict__ meta,
                                                       unsigned long long *__restrict__ err) {
     const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n_c) return;
@@ -59,6 +62,7 @@ __global__ void __launch_bounds__(256) k_scan_records(const uint64_t *__restrict
     SafeReader r{W, start, stop, start > stop || v >= n};
     uint64_t d = 0, n_itv = 0, n_res = 0, itv_len = 0;
     uint32_t si = 0, sr = 0;
+    uint64_t ibase = 0, rbase = 0;
     uint64_t block_end = start;
     for (int blk = 0; blk < 2 && !r.bad; blk++) {
         r.pos = block_end;
@@ -66,7 +70,7 @@ __global__ void __launch_bounds__(256) k_scan_records(const uint64_t *__restrict
         const uint32_t ns = r.gamma() + 1;
         const uint64_t base = r.pos;
         if (r.bad || base + (uint64_t)SEG_BITS * (ns - 1) >= stop + (ns > 1 ? 0 : 1)) { r.bad = true; break; }
-        if (blk == 0) si = ns; else sr = ns;
+        if (blk == 0) { si = ns; ibase = base - start; } else { sr = ns; rbase = base - start; }
         for (uint32_t j = 0; j < ns && !r.bad; j++) {
             r.pos = base + (uint64_t)SEG_BITS * j;
             r.lim = j + 1 < ns ? r.pos + SEG_BITS : stop;
@@ -106,11 +110,13 @@ __global__ void __launch_bounds__(256) k_scan_records(const uint64_t *__restrict
         }
     }
     if (!r.bad && block_end != stop) r.bad = true;
+    if (rbase >= (1ull << 32)) r.bad = true;   // segment metadata holds 32-bit relative offsets
     if (r.bad) {
         atomicCAS(err, 0ull, (unsigned long long)p + 1);
         d = 0;
     }
     deg[p] = (uint32_t)d;
+    meta[p] = make_uint4((uint32_t)ibase, si, (uint32_t)rbase, sr);
     if (feat) {
         float *f = feat + p * NFEAT;
         f[0] = (float)d;
@@ -142,13 +148,16 @@ __device__ void copy_bits(const uint64_t *__restrict__ src, uint64_t s, uint64_t
 
 // new position q takes old position perm[q]: record bits, offset, vertex id.
 __global__ void __launch_bounds__(256) k_relayout(const uint64_t *__restrict__ W, const uint64_t *__restrict__ offs,
-                                                  const uint32_t *__restrict__ vid, const uint32_t *__restrict__ perm,
+                                                  const uint32_t *__restrict__ vid, const uint4 *__restrict__ meta,
+                                                  const uint32_t *__restrict__ perm,
                                                   const uint64_t *__restrict__ new_offs, uint64_t *new_W,
-                                                  uint32_t *__restrict__ new_vid, uint64_t n_c) {
+                                                  uint32_t *__restrict__ new_vid, uint4 *__restrict__ new_meta,
+                                                  uint64_t n_c) {
     const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n_c) return;
     const uint32_t p = perm[q];
     new_vid[q] = vid[p];
+    new_meta[q] = meta[p];
     copy_bits(W, offs[p], new_W, new_offs[q], offs[p + 1] - offs[p]);
 }
 
