@@ -92,6 +92,8 @@ struct HeavySet {
     bool csr_part = false;
     bool cms = false;
     uint32_t nh = 0, nchunks = 0, kb = 1;
+    uint32_t nlocal = 0, nchunked = 0;
+    DevBuf<uint32_t> local, chunked, slot;   // heavy-vertex indices: one workgroup each / chunked; slots
     DevBuf<uint32_t> pos, chunk_h;
     DevBuf<uint64_t> loff, chunk_b, toff;
     DevBuf<uint32_t> list;
@@ -312,11 +314,21 @@ void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
     h.chunk_h = hs.chunk_h.p;
     h.chunk_b = hs.chunk_b.p;
     h.cms = hs.cms ? 1 : 0;
+    h.slot = hs.slot.p;
     if (!as_csr) hipLaunchKernelGGL(k_heavy_decode, dim3(hs.nh), dim3(BLOCK), 0, stream, h);
+    if (hs.nlocal) {
+        if (hs.cms)
+            hipLaunchKernelGGL(k_heavy_local<1>, dim3(hs.nlocal), dim3(BLOCK),
+                               (size_t)cms_depth * cms_width * sizeof(uint32_t), stream, h, hs.local.p, hs.nlocal);
+        else
+            hipLaunchKernelGGL(k_heavy_local<0>, dim3(hs.nlocal), dim3(BLOCK), 2 * CHUNK * sizeof(uint64_t), stream,
+                               h, hs.local.p, hs.nlocal);
+    }
+    if (hs.nchunked == 0) return;
     for (uint32_t k0 = 0; k0 < K; k0 += hs.kb) {
         h.k0 = k0;
         h.kb = std::min(hs.kb, K - k0);
-        hipLaunchKernelGGL(k_heavy_clear, dim3(hs.nh), dim3(BLOCK), 0, stream, h);
+        hipLaunchKernelGGL(k_heavy_clear, dim3(hs.nchunked), dim3(BLOCK), 0, stream, h, hs.chunked.p, hs.nchunked);
         if (hs.cms) {
             const size_t sk = (size_t)cms_depth * cms_width * sizeof(uint32_t);
             hipLaunchKernelGGL(k_heavy_chunks<1>, dim3(hs.nchunks), dim3(BLOCK), sk, stream, h);
@@ -325,8 +337,9 @@ void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
             hipLaunchKernelGGL(k_heavy_chunks<0>, dim3(hs.nchunks), dim3(BLOCK), 2 * CHUNK * sizeof(uint64_t),
                                stream, h);
         }
-        const uint64_t fin = (uint64_t)hs.nh * h.kb;
-        hipLaunchKernelGGL(k_heavy_finish, dim3((unsigned)((fin + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, stream, h);
+        const uint64_t fin = (uint64_t)hs.nchunked * h.kb;
+        hipLaunchKernelGGL(k_heavy_finish, dim3((unsigned)((fin + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, stream, h,
+                           hs.chunked.p, hs.nchunked);
     }
 }
 
@@ -407,6 +420,8 @@ int lpa_engine_create(int device, lpa_engine **out) {
             (void)hipFuncSetAttribute((const void *)k_group<BLOCK, MODE_CMS, false>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
             (void)hipFuncSetAttribute((const void *)k_group<BLOCK, MODE_CMS, true>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
             (void)hipFuncSetAttribute((const void *)k_heavy_chunks<0>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+            (void)hipFuncSetAttribute((const void *)k_heavy_local<0>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+            (void)hipFuncSetAttribute((const void *)k_heavy_local<1>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
             (void)hipFuncSetAttribute((const void *)k_heavy_chunks<1>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
             (void)hipGetLastError();
         } catch (...) {
@@ -580,9 +595,12 @@ static void build_heavy(lpa_engine *e) {
             auto hs = std::make_unique<HeavySet>();
             hs->csr_part = part == 1;
             hs->cms = cms == 1;
-            std::vector<uint32_t> pos;
+            std::vector<uint32_t> pos, local, chunked, slot;
             std::vector<uint64_t> loff(1, 0), toff(1, 0), cb;
             std::vector<uint32_t> ch;
+            // one workgroup holds the whole count: exact up to CHUNK ids (LDS
+            // table of 2*CHUNK slots), CMS up to CMS_LOCAL ids (LDS sketch)
+            const uint64_t local_max = cms ? CMS_LOCAL : CHUNK;
             for (const Bin &b : e->bins) {
                 if (b.csr != hs->csr_part) continue;
                 if (b.kind != (cms ? LPA_BIN_CMS : LPA_BIN_GLOBAL)) continue;
@@ -591,32 +609,42 @@ static void build_heavy(lpa_engine *e) {
                                                     : e->deg_c[e->order_c[q]];
                     const uint32_t i = (uint32_t)pos.size();
                     pos.push_back(q);
-                    for (uint64_t c0 = 0; c0 < d; c0 += CHUNK) {
-                        ch.push_back(i);
-                        cb.push_back(c0);
-                    }
+                    const bool is_local = d <= local_max;
+                    slot.push_back(is_local ? 0u : (uint32_t)chunked.size());
+                    (is_local ? local : chunked).push_back(i);
+                    if (!is_local)
+                        for (uint64_t c0 = 0; c0 < d; c0 += CHUNK) {
+                            ch.push_back(i);
+                            cb.push_back(c0);
+                        }
                     loff.push_back(loff.back() + d);
-                    toff.push_back(toff.back() + (cms ? 0 : (1ull << ceil_log2(std::max<uint64_t>(2 * d, 2)))));
+                    toff.push_back(toff.back() +
+                                   (cms || is_local ? 0 : (1ull << ceil_log2(std::max<uint64_t>(2 * d, 2)))));
                 }
             }
             if (pos.empty()) continue;
             hs->nh = (uint32_t)pos.size();
             hs->nchunks = (uint32_t)ch.size();
+            hs->nlocal = (uint32_t)local.size();
+            hs->nchunked = (uint32_t)chunked.size();
             hs->pos.upload(pos.data(), pos.size(), e->stream);
             hs->loff.upload(loff.data(), loff.size(), e->stream);
+            hs->local.upload(local.data(), local.size(), e->stream);
+            hs->chunked.upload(chunked.data(), chunked.size(), e->stream);
+            hs->slot.upload(slot.data(), slot.size(), e->stream);
             hs->chunk_h.upload(ch.data(), ch.size(), e->stream);
             hs->chunk_b.upload(cb.data(), cb.size(), e->stream);
             if (!hs->csr_part) hs->list.alloc(loff.back());
             if (hs->cms) {
                 hs->kb = e->K;
-                hs->sketch.alloc(((uint64_t)e->cms_depth * e->cms_width) * hs->nh * hs->kb);
+                hs->sketch.alloc(((uint64_t)e->cms_depth * e->cms_width) * std::max<uint32_t>(hs->nchunked, 1) * hs->kb);
             } else {
                 const uint64_t per_vec = toff.back() * 8;
                 hs->kb = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(e->K, table_budget / std::max<uint64_t>(per_vec, 1)));
                 hs->toff.upload(toff.data(), toff.size(), e->stream);
-                hs->tab.alloc(toff.back() * hs->kb);
+                hs->tab.alloc(std::max<uint64_t>(toff.back(), 1) * hs->kb);
             }
-            hs->result.alloc((uint64_t)hs->nh * hs->kb);
+            hs->result.alloc((uint64_t)std::max<uint32_t>(hs->nchunked, 1) * hs->kb);
             e->heavy.push_back(std::move(hs));
         }
     }

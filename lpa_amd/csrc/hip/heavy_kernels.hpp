@@ -29,6 +29,8 @@
 namespace lpa {
 
 constexpr uint32_t CHUNK = 4096;
+// CMS vertices up to this degree are counted by one workgroup (LDS sketch).
+constexpr uint32_t CMS_LOCAL = 65536;
 
 struct HeavyArgs {
     StepArgs s;                                // words/offs/vid, row/nbr/cvid, labels, CMS hashes
@@ -47,6 +49,7 @@ struct HeavyArgs {
     const uint64_t *__restrict__ chunk_b;      // first list entry of each chunk (relative to the vertex)
     int cms;                                   // 1: CMS mode, 0: exact
     uint32_t k0, kb;                           // label vectors of this batch
+    const uint32_t *__restrict__ slot;         // chunked vertex i -> its sketch / result slot
 };
 
 __device__ __forceinline__ const uint32_t *heavy_ids(const HeavyArgs &h, uint32_t i) {
@@ -67,13 +70,15 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_decode(HeavyArgs h) {
     if (h.s.rec && threadIdx.x == 0) atomicAdd(&h.s.rec[h.s.vid[p]], (cycles() - t0) * BLOCK);
 }
 
-__global__ void __launch_bounds__(BLOCK) k_heavy_clear(HeavyArgs h) {
-    const uint32_t i = blockIdx.x;
-    if (i >= h.nh) return;
-    for (uint32_t k = threadIdx.x; k < h.kb; k += BLOCK) h.result[(uint64_t)i * h.kb + k] = 0;
+// chunked[] lists the heavy-vertex indices handled by the chunk kernels.
+__global__ void __launch_bounds__(BLOCK) k_heavy_clear(HeavyArgs h, const uint32_t *__restrict__ chunked,
+                                                       uint32_t nchunked) {
+    if (blockIdx.x >= nchunked) return;
+    const uint32_t i = chunked[blockIdx.x], sl = h.slot[i];
+    for (uint32_t k = threadIdx.x; k < h.kb; k += BLOCK) h.result[(uint64_t)sl * h.kb + k] = 0;
     if (h.cms) {
         const uint64_t cells = ((uint64_t)h.s.cms_depth << h.s.cms_logw) * h.kb;
-        uint32_t *sk = h.sketch + (uint64_t)i * cells;
+        uint32_t *sk = h.sketch + (uint64_t)sl * cells;
         for (uint64_t c = threadIdx.x; c < cells; c += BLOCK) sk[c] = 0;
     } else {
         for (uint64_t c = h.toff[i] * h.kb + threadIdx.x; c < h.toff[i + 1] * h.kb; c += BLOCK) h.tab[c] = EMPTY_SLOT;
@@ -108,7 +113,7 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_chunks(HeavyArgs h) {
     for (uint32_t kk = 0; kk < h.kb; kk++) {
         const uint32_t k = h.k0 + kk;
         uint64_t best = 0;
-        uint32_t *sk = h.cms ? h.sketch + ((uint64_t)i * h.kb + kk) * cells : nullptr;
+        uint32_t *sk = h.cms ? h.sketch + ((uint64_t)h.slot[i] * h.kb + kk) * cells : nullptr;
         uint64_t *tab = h.cms ? nullptr : h.tab + h.toff[i] * h.kb + kk * T;
         if (PASS == 1) {
             for (uint32_t x = threadIdx.x; x < cells; x += BLOCK) lsk[x] = 0;
@@ -156,7 +161,7 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_chunks(HeavyArgs h) {
             __syncthreads();
             if (threadIdx.x == 0) {
                 for (int w = 0; w < BLOCK / 64; w++) best = red[w] > best ? red[w] : best;
-                if (best) atomicMax(&h.result[(uint64_t)i * h.kb + kk], (unsigned long long)best);
+                if (best) atomicMax(&h.result[(uint64_t)h.slot[i] * h.kb + kk], (unsigned long long)best);
             }
         }
         __syncthreads();
@@ -167,11 +172,83 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_chunks(HeavyArgs h) {
     }
 }
 
-__global__ void __launch_bounds__(BLOCK) k_heavy_finish(HeavyArgs h) {
-    const uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+// Heavy vertices small enough for one workgroup's LDS (exact: degree <=
+// CHUNK, table of 2^ceil(log2 2d) slots; CMS: any degree up to the local
+// limit, sketch in LDS): one workgroup per vertex counts its whole list --
+// both CMS passes, or the exact table -- and writes the label itself, with
+// no HBM table, sketch or merge.  local[] lists the heavy-vertex indices.
+template <int CMS>
+__global__ void __launch_bounds__(BLOCK) k_heavy_local(HeavyArgs h, const uint32_t *__restrict__ local,
+                                                       uint32_t nlocal) {
+    extern __shared__ uint64_t lds[];
+    __shared__ uint64_t red[BLOCK / 64];
+    if (blockIdx.x >= nlocal) return;
+    const uint32_t i = local[blockIdx.x];
+    const uint32_t d = heavy_deg(h, i);
+    const uint32_t *ids = heavy_ids(h, i);
+    const StepArgs &a = h.s;
+    const uint64_t t0 = a.rec ? cycles() : 0;
+    const uint32_t p = h.pos[i];
+    const uint32_t v = h.csr ? a.cvid[p] : a.vid[p];
+    const uint32_t cells = a.cms_depth << a.cms_logw;
+    const uint32_t T = d ? 1u << (32 - __builtin_clz(2 * d - 1)) : 1u;
+    const uint32_t logT = 31 - __builtin_clz(T);
+    uint32_t *sk = (uint32_t *)lds;
+    uint64_t *tab = lds;
+    for (uint32_t k = 0; k < a.K; k++) {
+        uint64_t best = 0;
+        if (CMS) {
+            for (uint32_t x = threadIdx.x; x < cells; x += BLOCK) sk[x] = 0;
+            __syncthreads();
+            group_count<BLOCK>(a, ids, d, threadIdx.x, k, [&](uint32_t lab) {
+                const uint32_t cnt = fold_first_label<BLOCK>(lab);
+                if (cnt == 0) return;
+                for (uint32_t r = 0; r < a.cms_depth; r++) atomicAdd(&sk[(r << a.cms_logw) + cms_col(a, r, lab)], cnt);
+            });
+            __syncthreads();
+            group_count<BLOCK>(a, ids, d, threadIdx.x, k, [&](uint32_t lab) {
+                uint32_t est = UNL;
+                for (uint32_t r = 0; r < a.cms_depth; r++) {
+                    const uint32_t x = sk[(r << a.cms_logw) + cms_col(a, r, lab)];
+                    est = x < est ? x : est;
+                }
+                const uint64_t key = ((uint64_t)est << 32) | (uint32_t)~lab;
+                best = key > best ? key : best;
+            });
+        } else {
+            for (uint32_t x = threadIdx.x; x < T; x += BLOCK) tab[x] = EMPTY_SLOT;
+            __syncthreads();
+            group_count<BLOCK>(a, ids, d, threadIdx.x, k, [&](uint32_t lab) {
+                const uint32_t cnt = fold_first_label<BLOCK>(lab);
+                if (cnt == 0) return;
+                const uint32_t c = tab_add(tab, logT, lab, cnt);
+                const uint64_t key = ((uint64_t)c << 32) | (uint32_t)~lab;
+                best = key > best ? key : best;
+            });
+        }
+        best = max_u64_over<64>(best);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = best;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 0; w < BLOCK / 64; w++) best = red[w] > best ? red[w] : best;
+            const uint64_t idx = (uint64_t)v * a.K + k;
+            const uint32_t old = a.lin[idx];
+            const uint32_t nl = pick(best, old);
+            a.lout[idx] = nl;
+            if (nl != old) atomicAdd(a.changed, 1ull);
+        }
+        __syncthreads();
+    }
+    if (a.rec && threadIdx.x == 0) atomicAdd(&a.rec[v], (cycles() - t0) * BLOCK);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_heavy_finish(HeavyArgs h, const uint32_t *__restrict__ chunked,
+                                                        uint32_t nchunked) {
+    const uint64_t t0 = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
     bool ch = false;
-    if (t < (uint64_t)h.nh * h.kb) {
-        const uint32_t i = (uint32_t)(t / h.kb), k = h.k0 + (uint32_t)(t % h.kb);
+    if (t0 < (uint64_t)nchunked * h.kb) {
+        const uint32_t i = chunked[t0 / h.kb], k = h.k0 + (uint32_t)(t0 % h.kb);
+        const uint64_t t = (uint64_t)h.slot[i] * h.kb + (t0 % h.kb);
         const uint32_t p = h.pos[i];
         const uint32_t v = h.csr ? h.s.cvid[p] : h.s.vid[p];
         const uint64_t idx = (uint64_t)v * h.s.K + k;

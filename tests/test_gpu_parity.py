@@ -125,3 +125,31 @@ def test_autotune_keeps_results():
     assert np.array_equal(e.labels(), before) and ms > 0
     print("autotuned", best, ms)
     run_and_compare(cgr, 3, engine=e)
+
+
+def _hub_graph(seed=21):
+    """background of ~8 random neighbours per vertex plus hubs of degree
+    3,000 / 6,000 / 70,000, so both the one-workgroup and the chunked
+    heavy paths run, exact and sketched"""
+    rng = np.random.default_rng(seed)
+    n = 120000
+    src = [rng.integers(0, n, 4 * n)]
+    dst = [(src[0] + rng.integers(1, 200, 4 * n)) % n]
+    for hub, d in ((0, 70000), (1, 3000), (2, 6000)):
+        src.append(np.full(d, hub))
+        dst.append(rng.choice(np.arange(3, n), d, replace=False))
+    src = np.concatenate(src).astype(np.uint32)
+    dst = np.concatenate(dst).astype(np.uint32)
+    return L.Csr.from_edges(n, src, dst, True)
+
+
+def test_heavy_local_and_chunked_paths():
+    g = _hub_graph()
+    deg = g.degrees()
+    assert deg.max() > 65536 and ((deg > 2048) & (deg <= 4096)).any() and ((deg > 4096) & (deg < 65536)).any()
+    cgr = g.encode(0, 128)
+    run_and_compare(cgr, 4)                                  # exact: global bin above 2048
+    run_and_compare(cgr, 4, cms_thresh=1000, cms_depth=4, cms_width=512, cms_seed=5)
+    run_and_compare(cgr, 3, K=3, cms_thresh=2500, cms_depth=3, cms_width=1024, cms_seed=6)
+    run_and_compare(cgr, 3, K=2)
+    run_and_compare(g.encode(16, 128), 3, cms_thresh=1000, cms_depth=4, cms_width=512, cms_seed=8)
