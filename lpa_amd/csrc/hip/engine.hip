@@ -153,12 +153,7 @@ struct lpa_engine {
     std::vector<uint32_t> order_s;          // same for the small-vertex CSR
     uint64_t bin_counts[LPA_NBIN_KINDS] = {0};
     bool scheduled = false;
-    DevBuf<uint64_t> gtab;
-    uint64_t gcap = 0;
-    DevBuf<uint32_t> glist;
-    uint64_t glcap = 0;
     std::vector<std::unique_ptr<HeavySet>> heavy;
-    uint32_t gblocks = 0;
 
     // LP state
     uint32_t K = 1;
@@ -226,10 +221,6 @@ struct lpa_engine {
         a.cms_depth = cms_depth;
         a.cms_logw = ceil_log2(cms_width);
         std::memcpy(a.cms_ab, cms_ab, sizeof cms_ab);
-        a.gtab = gtab.p;
-        a.gcap = gcap;
-        a.glist = glist.p;
-        a.glcap = glcap;
         a.changed = counter.p;
         a.rec = rec;
         return a;
@@ -260,6 +251,36 @@ static int wave_groups(uint32_t logT, uint32_t quarter_max = 128) {
     return (1u << (logT - 1)) <= quarter_max ? 16 : 4;
 }
 
+// label vectors staged per row gather by the group kernels
+static int group_kb(uint32_t K) { return K >= 5 ? 8 : K >= 3 ? 4 : K == 2 ? 2 : 1; }
+
+template <int G, bool CSR, int KB> static const void *group_fn() { return (const void *)k_group<G, CSR, KB>; }
+
+template <int G>
+static void launch_group(bool as_csr, uint32_t K, dim3 grid, size_t lds, hipStream_t s, const StepArgs &a,
+                         uint32_t logT, uint32_t cap) {
+    const void *fn;
+    switch (group_kb(K) * 2 + (as_csr ? 1 : 0)) {
+    case 2: fn = group_fn<G, false, 1>(); break;
+    case 3: fn = group_fn<G, true, 1>(); break;
+    case 4: fn = group_fn<G, false, 2>(); break;
+    case 5: fn = group_fn<G, true, 2>(); break;
+    case 8: fn = group_fn<G, false, 4>(); break;
+    case 9: fn = group_fn<G, true, 4>(); break;
+    case 16: fn = group_fn<G, false, 8>(); break;
+    default: fn = group_fn<G, true, 8>(); break;
+    }
+    StepArgs args = a;
+    void *params[] = {&args, &logT, &cap};
+    (void)hipLaunchKernel(fn, grid, dim3(BLOCK), params, lds, s);
+}
+
+template <int G> static void allow_group_lds(int dyn) {
+    const void *fns[] = {group_fn<G, false, 1>(), group_fn<G, true, 1>(), group_fn<G, false, 2>(), group_fn<G, true, 2>(),
+                         group_fn<G, false, 4>(), group_fn<G, true, 4>(), group_fn<G, false, 8>(), group_fn<G, true, 8>()};
+    for (const void *f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+}
+
 void lpa_engine::launch_bin(const Bin &b, StepArgs a, bool as_csr) {
     a.begin = b.begin;
     a.end = b.end;
@@ -276,21 +297,13 @@ void lpa_engine::launch_bin(const Bin &b, StepArgs a, bool as_csr) {
         const int ng = wave_groups(logT, sched_used.quarter_max ? sched_used.quarter_max : 128);
         const size_t lds = group_lds(ng, logT);
         dim3 grid((count + ng - 1) / ng);
-        if (ng == 16) {
-            if (as_csr) hipLaunchKernelGGL((k_group<16, MODE_LDS, true>), grid, dim3(BLOCK), lds, stream, a, logT, cap);
-            else hipLaunchKernelGGL((k_group<16, MODE_LDS, false>), grid, dim3(BLOCK), lds, stream, a, logT, cap);
-        } else {
-            if (as_csr) hipLaunchKernelGGL((k_group<64, MODE_LDS, true>), grid, dim3(BLOCK), lds, stream, a, logT, cap);
-            else hipLaunchKernelGGL((k_group<64, MODE_LDS, false>), grid, dim3(BLOCK), lds, stream, a, logT, cap);
-        }
+        if (ng == 16) launch_group<16>(as_csr, K, grid, lds, stream, a, logT, cap);
+        else launch_group<64>(as_csr, K, grid, lds, stream, a, logT, cap);
         break;
     }
     case LPA_BIN_BLOCK: {
         const uint32_t logT = (uint32_t)b.param, cap = 1u << (logT - 1);
-        const size_t lds = group_lds(1, logT);
-        dim3 grid(count);
-        if (as_csr) hipLaunchKernelGGL((k_group<BLOCK, MODE_LDS, true>), grid, dim3(BLOCK), lds, stream, a, logT, cap);
-        else hipLaunchKernelGGL((k_group<BLOCK, MODE_LDS, false>), grid, dim3(BLOCK), lds, stream, a, logT, cap);
+        launch_group<BLOCK>(as_csr, K, dim3(count), group_lds(1, logT), stream, a, logT, cap);
         break;
     }
     default: break;   // CMS and GLOBAL bins run through launch_heavy
@@ -315,13 +328,16 @@ void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
     h.chunk_b = hs.chunk_b.p;
     h.cms = hs.cms ? 1 : 0;
     h.slot = hs.slot.p;
+    // CMS: sketches of up to CMS_KB label vectors side by side in LDS
+    const size_t cells = (size_t)cms_depth * cms_width;
+    h.kbs = (uint32_t)std::max<size_t>(1, std::min<size_t>({(size_t)K, (size_t)CMS_KB, (size_t)(lds_max - 20480) / (cells * 4)}));
     if (!as_csr) hipLaunchKernelGGL(k_heavy_decode, dim3(hs.nh), dim3(BLOCK), 0, stream, h);
     if (hs.nlocal) {
         if (hs.cms)
-            hipLaunchKernelGGL(k_heavy_local<1>, dim3(hs.nlocal), dim3(BLOCK),
-                               (size_t)cms_depth * cms_width * sizeof(uint32_t), stream, h, hs.local.p, hs.nlocal);
+            hipLaunchKernelGGL(k_heavy_local<1>, dim3(hs.nlocal), dim3(HBLOCK), h.kbs * cells * sizeof(uint32_t), stream,
+                               h, hs.local.p, hs.nlocal);
         else
-            hipLaunchKernelGGL(k_heavy_local<0>, dim3(hs.nlocal), dim3(BLOCK), 2 * CHUNK * sizeof(uint64_t), stream,
+            hipLaunchKernelGGL(k_heavy_local<0>, dim3(hs.nlocal), dim3(HBLOCK), 2 * CHUNK * sizeof(uint64_t), stream,
                                h, hs.local.p, hs.nlocal);
     }
     if (hs.nchunked == 0) return;
@@ -330,11 +346,11 @@ void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
         h.kb = std::min(hs.kb, K - k0);
         hipLaunchKernelGGL(k_heavy_clear, dim3(hs.nchunked), dim3(BLOCK), 0, stream, h, hs.chunked.p, hs.nchunked);
         if (hs.cms) {
-            const size_t sk = (size_t)cms_depth * cms_width * sizeof(uint32_t);
-            hipLaunchKernelGGL(k_heavy_chunks<1>, dim3(hs.nchunks), dim3(BLOCK), sk, stream, h);
-            hipLaunchKernelGGL(k_heavy_chunks<2>, dim3(hs.nchunks), dim3(BLOCK), 0, stream, h);
+            const size_t sk = std::min(h.kbs, h.kb) * cells * sizeof(uint32_t);
+            hipLaunchKernelGGL(k_heavy_chunks<1>, dim3(hs.nchunks), dim3(HBLOCK), sk, stream, h);
+            hipLaunchKernelGGL(k_heavy_chunks<2>, dim3(hs.nchunks), dim3(HBLOCK), 0, stream, h);
         } else {
-            hipLaunchKernelGGL(k_heavy_chunks<0>, dim3(hs.nchunks), dim3(BLOCK), 2 * CHUNK * sizeof(uint64_t),
+            hipLaunchKernelGGL(k_heavy_chunks<0>, dim3(hs.nchunks), dim3(HBLOCK), 2 * CHUNK * sizeof(uint64_t),
                                stream, h);
         }
         const uint64_t fin = (uint64_t)hs.nchunked * h.kb;
@@ -409,16 +425,10 @@ int lpa_engine_create(int device, lpa_engine **out) {
             HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
             e->counter.alloc(1);
             HIPCHK(hipHostMalloc((void **)&e->h_counter, sizeof(unsigned long long), hipHostMallocDefault));
-            e->gblocks = (uint32_t)std::max(1, e->num_cu * 2);
             const int dyn = (int)(e->lds_max - 20480);
-            (void)hipFuncSetAttribute((const void *)k_group<16, MODE_LDS, false>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
-            (void)hipFuncSetAttribute((const void *)k_group<16, MODE_LDS, true>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
-            (void)hipFuncSetAttribute((const void *)k_group<64, MODE_LDS, false>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
-            (void)hipFuncSetAttribute((const void *)k_group<64, MODE_LDS, true>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
-            (void)hipFuncSetAttribute((const void *)k_group<BLOCK, MODE_LDS, false>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
-            (void)hipFuncSetAttribute((const void *)k_group<BLOCK, MODE_LDS, true>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
-            (void)hipFuncSetAttribute((const void *)k_group<BLOCK, MODE_CMS, false>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
-            (void)hipFuncSetAttribute((const void *)k_group<BLOCK, MODE_CMS, true>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+            allow_group_lds<16>(dyn);
+            allow_group_lds<64>(dyn);
+            allow_group_lds<BLOCK>(dyn);
             (void)hipFuncSetAttribute((const void *)k_heavy_chunks<0>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
             (void)hipFuncSetAttribute((const void *)k_heavy_local<0>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
             (void)hipFuncSetAttribute((const void *)k_heavy_local<1>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
@@ -671,7 +681,6 @@ static void build_layout(lpa_engine *e, const std::vector<uint8_t> &kind_c, cons
     };
     e->bins.clear();
     std::memset(e->bin_counts, 0, sizeof e->bin_counts);
-    uint64_t gmax = 0, lmax = 0;
     for (int part = 0; part < 2; part++) {
         const bool csr = part == 1;
         const uint64_t cnt = csr ? e->n_s : e->n_c;
@@ -695,8 +704,6 @@ static void build_layout(lpa_engine *e, const std::vector<uint8_t> &kind_c, cons
             Bin b{kinds[x], sub[x], csr, (uint32_t)q, (uint32_t)r, deg(x)};
             e->bins.push_back(b);
             e->bin_counts[b.kind] += r - q;
-            if (b.kind == LPA_BIN_GLOBAL) gmax = std::max<uint64_t>(gmax, b.maxdeg);
-            if ((b.kind == LPA_BIN_GLOBAL || b.kind == LPA_BIN_CMS) && !csr) lmax = std::max<uint64_t>(lmax, b.maxdeg);
             q = r;
         }
         if (!csr) {
@@ -738,22 +745,6 @@ static void build_layout(lpa_engine *e, const std::vector<uint8_t> &kind_c, cons
         }
     }
     build_heavy(e);
-    gmax = 0;
-    lmax = 0;
-    if (gmax) {
-        e->gcap = 1ull << ceil_log2(2 * gmax);
-        e->gtab.alloc(e->gcap * e->gblocks);
-    } else {
-        e->gtab.release();
-        e->gcap = 0;
-    }
-    if (lmax) {
-        e->glcap = (lmax + 63) & ~63ull;
-        e->glist.alloc(e->glcap * e->gblocks);
-    } else {
-        e->glist.release();
-        e->glcap = 0;
-    }
     e->have_csr = false;
     e->scheduled = true;
     (void)p;

@@ -29,6 +29,9 @@
 namespace lpa {
 
 constexpr uint32_t CHUNK = 4096;
+// workgroup size of the chunk and one-workgroup heavy kernels (large LDS
+// tables and sketches; many waves share them)
+constexpr int HBLOCK = 1024;
 // CMS vertices up to this degree are counted by one workgroup (LDS sketch).
 constexpr uint32_t CMS_LOCAL = 65536;
 
@@ -49,6 +52,7 @@ struct HeavyArgs {
     const uint64_t *__restrict__ chunk_b;      // first list entry of each chunk (relative to the vertex)
     int cms;                                   // 1: CMS mode, 0: exact
     uint32_t k0, kb;                           // label vectors of this batch
+    uint32_t kbs;                              // CMS: label vectors per sketch pass (sketches side by side in LDS)
     const uint32_t *__restrict__ slot;         // chunked vertex i -> its sketch / result slot
 };
 
@@ -85,16 +89,84 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_clear(HeavyArgs h, const uint32
     }
 }
 
-// PASS 1: sketch increments, 2: sketch queries, 0: exact counting.
-// Passes 1 and 0 first aggregate the chunk in LDS (a chunk-local sketch, or
-// a chunk-local hash table of 2*CHUNK slots) and then merge only the
-// non-zero cells / occupied slots into the vertex's HBM sketch or table, so
-// a hub whose neighbours share a few labels does not funnel every arc into
-// the same HBM atomic.  Dynamic LDS: the local sketch or table.
+// Block-wide maximum (every lane gets it); red[] is free again on return.
+__device__ __forceinline__ uint64_t block_max(uint64_t x, uint64_t *red) {
+    x = max_u64_over<64>(x);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < HBLOCK / 64; w++) x = red[w] > x ? red[w] : x;
+    __syncthreads();
+    return x;
+}
+
+// Exact counting stages the label rows of up to CHUNK neighbours per
+// workgroup in registers (HEAVY_IPL per lane), HEAVY_KB vectors at a time.
+constexpr int HEAVY_IPL = CHUNK / HBLOCK;
+constexpr int HEAVY_KB = 8;
+// Sketch passes walk the list in rounds of CMS_R neighbours per lane and
+// serve up to CMS_KB label vectors (kbs sketches side by side) per round.
+constexpr int CMS_R = 4;
+constexpr int CMS_KB = 8;
+
+// One sketch pass over ids[0..n) for vectors k0 .. k0+kb-1 (kb <= CMS_KB;
+// sketch s at sk + s*cells).  ADD: add every label to its sketch (LDS,
+// pre-aggregated per wave); else: estimate every label from its sketch and
+// keep the best (count << 32 | ~label) per vector in best[s].
+template <bool ADD>
+__device__ __forceinline__ void cms_rounds(const StepArgs &a, const uint32_t *__restrict__ ids, uint32_t n,
+                                           uint32_t k0, uint32_t kb, uint32_t *sk, uint32_t cells,
+                                           uint64_t (&best)[CMS_KB]) {
+    for (uint32_t i0 = 0; i0 < n; i0 += CMS_R * HBLOCK) {
+        uint32_t labs[CMS_R][CMS_KB];
+#pragma unroll
+        for (int t = 0; t < CMS_R; t++) {
+            const uint32_t i = i0 + t * HBLOCK + threadIdx.x;
+            if (i < n) {
+                load_row<CMS_KB>(a, ids[i], k0, kb, labs[t]);
+            } else {
+#pragma unroll
+                for (int q = 0; q < CMS_KB; q++) labs[t][q] = UNL;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < CMS_KB; q++) {
+            if ((uint32_t)q >= kb) continue;
+            uint32_t *skq = sk + (uint64_t)q * cells;
+#pragma unroll
+            for (int t = 0; t < CMS_R; t++) {
+                const uint32_t lab = labs[t][q];
+                if (lab == UNL) continue;
+                if (ADD) {
+                    const uint32_t cnt = fold_first_label<HBLOCK>(lab);
+                    if (cnt == 0) continue;
+                    for (uint32_t r = 0; r < a.cms_depth; r++)
+                        atomicAdd(&skq[(r << a.cms_logw) + cms_col(a, r, lab)], cnt);
+                } else {
+                    uint32_t est = UNL;
+                    for (uint32_t r = 0; r < a.cms_depth; r++) {
+                        const uint32_t x = skq[(r << a.cms_logw) + cms_col(a, r, lab)];
+                        est = x < est ? x : est;
+                    }
+                    const uint64_t key = ((uint64_t)est << 32) | (uint32_t)~lab;
+                    best[q] = key > best[q] ? key : best[q];
+                }
+            }
+        }
+    }
+}
+
+// PASS 1: sketch increments, 2: sketch queries, 0: exact counting, for the
+// label vectors [k0, k0+kb) of one chunk.  Passes 1 and 0 first aggregate
+// the chunk in LDS (chunk-local sketches, or a chunk-local hash table of
+// 2*CHUNK slots) and then merge only the non-zero cells / occupied slots
+// into the vertex's HBM sketch or table, so a hub whose neighbours share a
+// few labels does not funnel every arc into the same HBM atomic.  Dynamic
+// LDS: kbs local sketches (pass 1) or the local table (pass 0).
 template <int PASS>
-__global__ void __launch_bounds__(BLOCK) k_heavy_chunks(HeavyArgs h) {
+__global__ void __launch_bounds__(HBLOCK) k_heavy_chunks(HeavyArgs h) {
     extern __shared__ uint64_t lds[];
-    __shared__ uint64_t red[BLOCK / 64];
+    __shared__ uint64_t red[HBLOCK / 64];
     const uint32_t c = blockIdx.x;
     if (c >= h.nchunks) return;
     const uint32_t i = h.chunk_h[c];
@@ -103,85 +175,87 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_chunks(HeavyArgs h) {
     const uint32_t n = (uint32_t)((d - b) < CHUNK ? (d - b) : CHUNK);
     const uint32_t *ids = heavy_ids(h, i) + b;
     const StepArgs &a = h.s;
-    const uint64_t T = h.cms ? 0 : h.toff[i + 1] - h.toff[i];
-    const uint32_t logT = h.cms ? 0 : 63 - __builtin_clzll(T);
-    const uint32_t cells = a.cms_depth << a.cms_logw;
-    constexpr uint32_t LT = 2 * CHUNK, LOG_LT = 31 - __builtin_clz(LT);
-    uint32_t *lsk = (uint32_t *)lds;   // PASS 1
-    uint64_t *ltab = lds;              // PASS 0
     const uint64_t t0 = a.rec ? cycles() : 0;
-    for (uint32_t kk = 0; kk < h.kb; kk++) {
-        const uint32_t k = h.k0 + kk;
-        uint64_t best = 0;
-        uint32_t *sk = h.cms ? h.sketch + ((uint64_t)h.slot[i] * h.kb + kk) * cells : nullptr;
-        uint64_t *tab = h.cms ? nullptr : h.tab + h.toff[i] * h.kb + kk * T;
-        if (PASS == 1) {
-            for (uint32_t x = threadIdx.x; x < cells; x += BLOCK) lsk[x] = 0;
-        } else if (PASS == 0) {
-            for (uint32_t x = threadIdx.x; x < LT; x += BLOCK) ltab[x] = EMPTY_SLOT;
+    const uint32_t sl = h.slot[i];
+    if (PASS == 0) {
+        const uint64_t T = h.toff[i + 1] - h.toff[i];
+        const uint32_t logT = 63 - __builtin_clzll(T);
+        constexpr uint32_t LT = 2 * CHUNK, LOG_LT = 31 - __builtin_clz(LT);
+        uint64_t *ltab = lds;
+        for (uint32_t kq = 0; kq < h.kb; kq += HEAVY_KB) {
+            const uint32_t kb = h.kb - kq < (uint32_t)HEAVY_KB ? h.kb - kq : (uint32_t)HEAVY_KB;
+            uint64_t best = 0;
+            count_staged<HBLOCK, HEAVY_IPL, HEAVY_KB>(
+                a, ids, n, threadIdx.x, h.k0 + kq, kb,
+                [&](int) {
+                    for (uint32_t x = threadIdx.x; x < LT; x += HBLOCK) ltab[x] = EMPTY_SLOT;
+                    best = 0;
+                    __syncthreads();
+                },
+                [&](int, uint32_t lab) {
+                    const uint32_t cnt = fold_first_label<HBLOCK>(lab);
+                    if (cnt) (void)tab_add(ltab, LOG_LT, lab, cnt);
+                },
+                [&](int s) {
+                    __syncthreads();
+                    uint64_t *tab = h.tab + h.toff[i] * h.kb + (uint64_t)(kq + s) * T;
+                    for (uint32_t x = threadIdx.x; x < LT; x += HBLOCK) {
+                        const uint64_t e = ltab[x];
+                        if ((uint32_t)(e >> 32) == UNL) continue;
+                        const uint32_t lab = (uint32_t)(e >> 32);
+                        const uint32_t cc = tab_add(tab, logT, lab, (uint32_t)e);
+                        const uint64_t key = ((uint64_t)cc << 32) | (uint32_t)~lab;
+                        best = key > best ? key : best;
+                    }
+                    best = block_max(best, red);
+                    if (threadIdx.x == 0 && best)
+                        atomicMax(&h.result[(uint64_t)sl * h.kb + kq + s], (unsigned long long)best);
+                });
         }
-        __syncthreads();
-        group_count<BLOCK>(a, ids, n, threadIdx.x, k, [&](uint32_t lab) {
-            if (PASS == 2) {
-                uint32_t est = UNL;
-                for (uint32_t r = 0; r < a.cms_depth; r++) {
-                    uint32_t x = sk[(r << a.cms_logw) + cms_col(a, r, lab)];
-                    est = x < est ? x : est;
-                }
-                uint64_t key = ((uint64_t)est << 32) | (uint32_t)~lab;
-                best = key > best ? key : best;
-                return;
-            }
-            const uint32_t cnt = fold_first_label<BLOCK>(lab);
-            if (cnt == 0) return;
+    } else {
+        const uint32_t cells = a.cms_depth << a.cms_logw;
+        for (uint32_t kq = 0; kq < h.kb; kq += h.kbs) {
+            const uint32_t kb = h.kb - kq < h.kbs ? h.kb - kq : h.kbs;
+            uint32_t *hsk = h.sketch + ((uint64_t)sl * h.kb + kq) * cells;
+            uint64_t best[CMS_KB] = {};
             if (PASS == 1) {
-                for (uint32_t r = 0; r < a.cms_depth; r++)
-                    atomicAdd(&lsk[(r << a.cms_logw) + cms_col(a, r, lab)], cnt);
+                uint32_t *lsk = (uint32_t *)lds;
+                for (uint32_t x = threadIdx.x; x < kb * cells; x += HBLOCK) lsk[x] = 0;
+                __syncthreads();
+                cms_rounds<true>(a, ids, n, h.k0 + kq, kb, lsk, cells, best);
+                __syncthreads();
+                for (uint32_t x = threadIdx.x; x < kb * cells; x += HBLOCK)
+                    if (lsk[x]) atomicAdd(&hsk[x], lsk[x]);
+                __syncthreads();
             } else {
-                (void)tab_add(ltab, LOG_LT, lab, cnt);
-            }
-        });
-        __syncthreads();
-        if (PASS == 1) {
-            for (uint32_t x = threadIdx.x; x < cells; x += BLOCK)
-                if (lsk[x]) atomicAdd(&sk[x], lsk[x]);
-        } else if (PASS == 0) {
-            for (uint32_t x = threadIdx.x; x < LT; x += BLOCK) {
-                const uint64_t e = ltab[x];
-                if ((uint32_t)(e >> 32) == UNL) continue;
-                const uint32_t lab = (uint32_t)(e >> 32);
-                const uint32_t cc = tab_add(tab, logT, lab, (uint32_t)e);
-                uint64_t key = ((uint64_t)cc << 32) | (uint32_t)~lab;
-                best = key > best ? key : best;
+                cms_rounds<false>(a, ids, n, h.k0 + kq, kb, hsk, cells, best);
+#pragma unroll
+                for (int q = 0; q < CMS_KB; q++) {
+                    if ((uint32_t)q >= kb) continue;
+                    const uint64_t m = block_max(best[q], red);
+                    if (threadIdx.x == 0 && m)
+                        atomicMax(&h.result[(uint64_t)sl * h.kb + kq + q], (unsigned long long)m);
+                }
             }
         }
-        if (PASS != 1) {
-            best = max_u64_over<64>(best);
-            if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = best;
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                for (int w = 0; w < BLOCK / 64; w++) best = red[w] > best ? red[w] : best;
-                if (best) atomicMax(&h.result[(uint64_t)h.slot[i] * h.kb + kk], (unsigned long long)best);
-            }
-        }
-        __syncthreads();
     }
     if (a.rec && threadIdx.x == 0) {
         const uint32_t p = h.pos[i];
-        atomicAdd(&a.rec[h.csr ? a.cvid[p] : a.vid[p]], (cycles() - t0) * BLOCK);
+        atomicAdd(&a.rec[h.csr ? a.cvid[p] : a.vid[p]], (cycles() - t0) * HBLOCK);
     }
 }
 
 // Heavy vertices small enough for one workgroup's LDS (exact: degree <=
 // CHUNK, table of 2^ceil(log2 2d) slots; CMS: any degree up to the local
-// limit, sketch in LDS): one workgroup per vertex counts its whole list --
-// both CMS passes, or the exact table -- and writes the label itself, with
-// no HBM table, sketch or merge.  local[] lists the heavy-vertex indices.
+// limit, kbs sketches in LDS): one workgroup per vertex counts its whole
+// list -- both CMS passes, or the exact table -- and writes the labels
+// itself, with no HBM table, sketch or merge.  local[] lists the
+// heavy-vertex indices.
 template <int CMS>
-__global__ void __launch_bounds__(BLOCK) k_heavy_local(HeavyArgs h, const uint32_t *__restrict__ local,
+__global__ void __launch_bounds__(HBLOCK) k_heavy_local(HeavyArgs h, const uint32_t *__restrict__ local,
                                                        uint32_t nlocal) {
     extern __shared__ uint64_t lds[];
-    __shared__ uint64_t red[BLOCK / 64];
+    __shared__ uint64_t red[HBLOCK / 64];
     if (blockIdx.x >= nlocal) return;
     const uint32_t i = local[blockIdx.x];
     const uint32_t d = heavy_deg(h, i);
@@ -190,56 +264,61 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_local(HeavyArgs h, const uint32
     const uint64_t t0 = a.rec ? cycles() : 0;
     const uint32_t p = h.pos[i];
     const uint32_t v = h.csr ? a.cvid[p] : a.vid[p];
-    const uint32_t cells = a.cms_depth << a.cms_logw;
-    const uint32_t T = d ? 1u << (32 - __builtin_clz(2 * d - 1)) : 1u;
-    const uint32_t logT = 31 - __builtin_clz(T);
-    uint32_t *sk = (uint32_t *)lds;
-    uint64_t *tab = lds;
-    for (uint32_t k = 0; k < a.K; k++) {
-        uint64_t best = 0;
-        if (CMS) {
-            for (uint32_t x = threadIdx.x; x < cells; x += BLOCK) sk[x] = 0;
+    uint32_t nch = 0;
+    auto finish = [&](uint32_t k, uint64_t best) {   // thread 0
+        const uint64_t idx = (uint64_t)v * a.K + k;
+        const uint32_t old = a.lin[idx];
+        const uint32_t nl = pick(best, old);
+        a.lout[idx] = nl;
+        nch += nl != old;
+    };
+    if (CMS) {
+        const uint32_t cells = a.cms_depth << a.cms_logw;
+        uint32_t *sk = (uint32_t *)lds;
+        for (uint32_t k0 = 0; k0 < a.K; k0 += h.kbs) {
+            const uint32_t kb = a.K - k0 < h.kbs ? a.K - k0 : h.kbs;
+            for (uint32_t x = threadIdx.x; x < kb * cells; x += HBLOCK) sk[x] = 0;
             __syncthreads();
-            group_count<BLOCK>(a, ids, d, threadIdx.x, k, [&](uint32_t lab) {
-                const uint32_t cnt = fold_first_label<BLOCK>(lab);
-                if (cnt == 0) return;
-                for (uint32_t r = 0; r < a.cms_depth; r++) atomicAdd(&sk[(r << a.cms_logw) + cms_col(a, r, lab)], cnt);
-            });
+            uint64_t best[CMS_KB] = {};
+            cms_rounds<true>(a, ids, d, k0, kb, sk, cells, best);
             __syncthreads();
-            group_count<BLOCK>(a, ids, d, threadIdx.x, k, [&](uint32_t lab) {
-                uint32_t est = UNL;
-                for (uint32_t r = 0; r < a.cms_depth; r++) {
-                    const uint32_t x = sk[(r << a.cms_logw) + cms_col(a, r, lab)];
-                    est = x < est ? x : est;
-                }
-                const uint64_t key = ((uint64_t)est << 32) | (uint32_t)~lab;
-                best = key > best ? key : best;
-            });
-        } else {
-            for (uint32_t x = threadIdx.x; x < T; x += BLOCK) tab[x] = EMPTY_SLOT;
-            __syncthreads();
-            group_count<BLOCK>(a, ids, d, threadIdx.x, k, [&](uint32_t lab) {
-                const uint32_t cnt = fold_first_label<BLOCK>(lab);
-                if (cnt == 0) return;
-                const uint32_t c = tab_add(tab, logT, lab, cnt);
-                const uint64_t key = ((uint64_t)c << 32) | (uint32_t)~lab;
-                best = key > best ? key : best;
-            });
+            cms_rounds<false>(a, ids, d, k0, kb, sk, cells, best);
+#pragma unroll
+            for (int q = 0; q < CMS_KB; q++) {
+                if ((uint32_t)q >= kb) continue;
+                const uint64_t m = block_max(best[q], red);
+                if (threadIdx.x == 0) finish(k0 + q, m);
+            }
         }
-        best = max_u64_over<64>(best);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = best;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int w = 0; w < BLOCK / 64; w++) best = red[w] > best ? red[w] : best;
-            const uint64_t idx = (uint64_t)v * a.K + k;
-            const uint32_t old = a.lin[idx];
-            const uint32_t nl = pick(best, old);
-            a.lout[idx] = nl;
-            if (nl != old) atomicAdd(a.changed, 1ull);
+    } else {
+        const uint32_t T = d ? 1u << (32 - __builtin_clz(2 * d - 1)) : 1u;
+        const uint32_t logT = 31 - __builtin_clz(T);
+        uint64_t *tab = lds;
+        for (uint32_t k0 = 0; k0 < a.K; k0 += HEAVY_KB) {
+            const uint32_t kb = a.K - k0 < (uint32_t)HEAVY_KB ? a.K - k0 : (uint32_t)HEAVY_KB;
+            uint64_t best = 0;
+            count_staged<HBLOCK, HEAVY_IPL, HEAVY_KB>(
+                a, ids, d, threadIdx.x, k0, kb,
+                [&](int) {
+                    for (uint32_t x = threadIdx.x; x < T; x += HBLOCK) tab[x] = EMPTY_SLOT;
+                    best = 0;
+                    __syncthreads();
+                },
+                [&](int, uint32_t lab) {
+                    const uint32_t cnt = fold_first_label<HBLOCK>(lab);
+                    if (cnt == 0) return;
+                    const uint32_t c = tab_add(tab, logT, lab, cnt);
+                    const uint64_t key = ((uint64_t)c << 32) | (uint32_t)~lab;
+                    best = key > best ? key : best;
+                },
+                [&](int s) {
+                    const uint64_t m = block_max(best, red);
+                    if (threadIdx.x == 0) finish(k0 + s, m);
+                });
         }
-        __syncthreads();
     }
-    if (a.rec && threadIdx.x == 0) atomicAdd(&a.rec[v], (cycles() - t0) * BLOCK);
+    if (threadIdx.x == 0 && nch) atomicAdd(a.changed, (unsigned long long)nch);
+    if (a.rec && threadIdx.x == 0) atomicAdd(&a.rec[v], (cycles() - t0) * HBLOCK);
 }
 
 __global__ void __launch_bounds__(BLOCK) k_heavy_finish(HeavyArgs h, const uint32_t *__restrict__ chunked,

@@ -10,16 +10,18 @@
 // Here every shape computes the exact LP-SC result (double-buffered labels,
 // most frequent label, ties to the smallest) and shapes differ only in
 // speed:
-//   k_lane   one lane per vertex (deg <= 32): serial decode into an LDS
-//            list, labels gathered into registers, bitonic sorting network,
-//            longest run.
-//   k_group  G lanes per vertex (G = 64: one wave, G = 256: one workgroup):
-//            interval and residual segments (256-bit units) decoded one per
-//            lane, each decoded neighbour's label counted in an open-address
-//            hash table (LDS, or HBM for very large degree) with 64-bit
-//            atomics on (label << 32 | count) words; or, for vertices above
-//            the CMS threshold, a count-min sketch in LDS filled in one
-//            decode pass and queried in a second.
+//   k_lane   one lane per vertex (deg <= 32) decodes into an LDS list;
+//            then one lane per (vertex, label vector) pair gathers the
+//            labels into registers, sorts them with a bitonic network and
+//            takes the longest run.
+//   k_group  G lanes per vertex (G = 16: a quarter wave, 64: one wave,
+//            256: one workgroup): segments (256-bit units) decoded one per
+//            lane into an LDS list, then each neighbour's label row gathered
+//            once per batch of label vectors and counted, vector by vector,
+//            in an open-address LDS hash table with 64-bit atomics on
+//            (label << 32 | count) words.
+// Vertices above the count-min-sketch threshold and vertices too large for
+// a workgroup's LDS table run through the heavy-vertex kernels.
 // A lane keeps max((count << 32) | ~label) over every count it produced, so
 // the group maximum names the most frequent label and, among equal counts,
 // the smallest one, without a pass over the table.
@@ -57,12 +59,6 @@ struct StepArgs {
     // CMS (row hashes a_r, b_r)
     uint32_t cms_depth, cms_logw;
     uint64_t cms_ab[16];
-    // scratch in HBM for the largest degrees: per workgroup gcap table slots
-    // and glcap neighbour ids
-    uint64_t *__restrict__ gtab;
-    uint64_t gcap;
-    uint32_t *__restrict__ glist;
-    uint64_t glcap;
     unsigned long long *__restrict__ changed;
     uint32_t begin, end;                   // position range of this bin
     // work recorder (null: off): lane-cycles spent per vertex id
@@ -80,6 +76,19 @@ __device__ __forceinline__ uint32_t pick(uint64_t best, uint32_t old) {
     // best == 0: no labelled neighbour -> keep; seeds stay clamped
     if ((old != UNL && (old & SEED_BIT)) || best == 0) return old;
     return ~(uint32_t)best;
+}
+
+// Adds the block's changed labels to the counter with one atomic: n is the
+// calling lane's count; every thread of the block must call it.
+__device__ __forceinline__ void flush_changes(uint32_t n, unsigned long long *counter) {
+    __shared__ uint32_t total;
+    if (threadIdx.x == 0) total = 0;
+    __syncthreads();
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o, 64);
+    if ((threadIdx.x & 63) == 0 && n) atomicAdd(&total, n);
+    __syncthreads();
+    if (threadIdx.x == 0 && total) atomicAdd(counter, (unsigned long long)total);
 }
 
 __device__ __forceinline__ void count_changes(bool ch, unsigned long long *counter) {
@@ -121,54 +130,92 @@ __device__ __forceinline__ uint64_t longest_run(const uint32_t (&x)[N]) {
     return bestc ? ((uint64_t)bestc << 32) | (uint32_t)~best : 0;
 }
 
-// CSR=false: neighbours decoded from the CGR record; CSR=true: read from row/nbr.
+// Loads the labels of vectors k0 .. k0+kb-1 (kb <= KB) of vertex u, cleaned
+// (UNL beyond kb): the K labels of a vertex are contiguous, so one gather
+// brings a whole row slice (16-byte loads when K and k0 are multiples of 4).
+template <int KB>
+__device__ __forceinline__ void load_row(const StepArgs &a, uint32_t u, uint32_t k0, uint32_t kb,
+                                         uint32_t (&x)[KB]) {
+    const uint32_t *src = a.lin + (uint64_t)u * a.K + k0;
+    if (KB % 4 == 0 && kb == (uint32_t)KB && ((a.K | k0) & 3) == 0) {
+#pragma unroll
+        for (int q = 0; q < KB / 4; q++) {
+            const uint4 w = reinterpret_cast<const uint4 *>(src)[q];
+            x[4 * q] = clean_label(w.x);
+            x[4 * q + 1] = clean_label(w.y);
+            x[4 * q + 2] = clean_label(w.z);
+            x[4 * q + 3] = clean_label(w.w);
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < KB; t++) x[t] = (uint32_t)t < kb ? clean_label(src[t]) : UNL;
+    }
+}
+
+// One lane per vertex decodes the record into LDS (CSR=false) or looks up
+// its CSR row (CSR=true); then the block's (vertex, label vector) pairs are
+// dealt out to the lanes with the vector index fastest, so the lanes that
+// gather neighbour u's labels for consecutive vectors read consecutive
+// words of u's label row in one access.  Each lane sorts its pair's labels
+// with a bitonic network and takes the longest run.
 template <int N, bool CSR>
 __global__ void __launch_bounds__(BLOCK) k_lane(StepArgs a) {
     __shared__ uint32_t list[CSR ? 1 : N][BLOCK];
-    const uint32_t p = a.begin + blockIdx.x * BLOCK + threadIdx.x;
-    const bool active = p < a.end;
+    __shared__ uint32_t sv[BLOCK], sd[BLOCK];
+    __shared__ uint64_t sbase[CSR ? BLOCK : 1];
+    const uint32_t first = a.begin + blockIdx.x * BLOCK;
+    const uint32_t p = first + threadIdx.x;
+    const uint32_t nact = a.end - first < (uint32_t)BLOCK ? a.end - first : (uint32_t)BLOCK;
     const uint64_t t0 = a.rec ? cycles() : 0;
-    uint32_t v = 0, d = 0;
-    uint64_t base = 0;
-    if (active) {
+    if (p < a.end) {
+        uint32_t d = 0;
         if (CSR) {
-            v = a.cvid[p];
-            base = a.row[p];
-            d = (uint32_t)(a.row[p + 1] - base);
+            sv[threadIdx.x] = a.cvid[p];
+            sbase[threadIdx.x] = a.row[p];
+            d = (uint32_t)(a.row[p + 1] - a.row[p]);
         } else {
-            v = a.vid[p];
+            const uint32_t v = a.vid[p];
+            sv[threadIdx.x] = v;
             decode_record(a.words, a.offs[p], v, [&](uint32_t u) {
                 if (d < N) list[d][threadIdx.x] = u;
                 d++;
             });
         }
+        sd[threadIdx.x] = d;
+        if (a.rec) atomicAdd(&a.rec[sv[threadIdx.x]], cycles() - t0);
     }
-    for (uint32_t k = 0; k < a.K; k++) {
+    __syncthreads();
+    const uint32_t K = a.K, pairs = nact * K;
+    uint32_t nch = 0;
+    for (uint32_t f0 = 0; f0 < pairs; f0 += BLOCK) {
+        const uint64_t t1 = a.rec ? cycles() : 0;
+        const uint32_t f = f0 + threadIdx.x;
+        const bool active = f < pairs;
+        const uint32_t vb = active ? f / K : 0, k = active ? f - vb * K : 0;
+        const uint32_t d = active ? sd[vb] : 0;
         uint32_t x[N];
 #pragma unroll
         for (int j = 0; j < N; j++) {
             uint32_t u = 0;
-            if ((uint32_t)j < d) u = CSR ? a.nbr[base + j] : list[CSR ? 0 : j][threadIdx.x];
-            x[j] = (uint32_t)j < d ? clean_label(a.lin[(uint64_t)u * a.K + k]) : UNL;
+            if ((uint32_t)j < d) u = CSR ? a.nbr[sbase[CSR ? vb : 0] + j] : list[CSR ? 0 : j][vb];
+            x[j] = (uint32_t)j < d ? clean_label(a.lin[(uint64_t)u * K + k]) : UNL;
         }
         bitonic_sort<N>(x);
-        uint64_t best = longest_run<N>(x);
-        bool ch = false;
+        const uint64_t best = longest_run<N>(x);
         if (active) {
-            uint64_t idx = (uint64_t)v * a.K + k;
-            uint32_t old = a.lin[idx];
-            uint32_t nl = pick(best, old);
+            const uint32_t v = sv[vb];
+            const uint64_t idx = (uint64_t)v * K + k;
+            const uint32_t old = a.lin[idx];
+            const uint32_t nl = pick(best, old);
             a.lout[idx] = nl;
-            ch = nl != old;
+            nch += nl != old;
+            if (a.rec) atomicAdd(&a.rec[v], cycles() - t1);
         }
-        count_changes(ch, a.changed);
     }
-    if (a.rec && active) a.rec[v] = cycles() - t0;
+    flush_changes(nch, a.changed);
 }
 
 // ------------------------------------------------------------ group kernels
-
-enum { MODE_LDS = 0, MODE_GLOBAL = 1, MODE_CMS = 2 };
 
 // Groups of G lanes: G = 16 (four vertices per wave), 64 (one wave) or 256
 // (one workgroup).  Sub-wave groups synchronise with the wave.
@@ -241,11 +288,6 @@ __device__ __forceinline__ uint32_t cms_col(const StepArgs &a, uint32_t r, uint3
 
 // Intervals at least this long are written by the whole group.
 constexpr uint32_t LONG_RUN = 16;
-// Labels gathered per lane before they are counted.
-#ifndef LPA_UNROLL
-#define LPA_UNROLL 4
-#endif
-constexpr int UNROLL = LPA_UNROLL;
 
 template <int G> struct GroupShared {
     static constexpr int QCAP = G == BLOCK ? 512 : G == 64 ? 64 : 16;
@@ -314,30 +356,55 @@ __device__ __forceinline__ uint32_t group_decode(const StepArgs &a, uint32_t p, 
     return sh->cnt;
 }
 
-// Phase B: lane gl of the group walks ids gl, gl+G, ... of the list,
-// gathering UNROLL labels before counting them.
-template <int G, class F>
-__device__ __forceinline__ void group_count(const StepArgs &a, const uint32_t *__restrict__ ids, uint32_t d,
-                                            uint32_t gl, uint32_t k, F &&use) {
-    for (uint32_t i0 = 0; i0 < d; i0 += UNROLL * G) {
-        uint32_t labs[UNROLL];
+// Phase B: counts label vectors k0 .. k0+kb-1 (kb <= KB, the same in every
+// lane of the group) of the neighbours ids[0..d), one vector at a time.
+// Lane gl first gathers the label-row slices of neighbours gl, gl+G, ...,
+// gl+(IPL-1)G into registers (IPL*KB values, all loads in flight together),
+// so a vertex's neighbour rows are read once per batch of KB vectors;
+// neighbours beyond IPL*G are gathered per vector.  For each vector s:
+// begin(s) prepares the table (and synchronises the group), use(s, lab)
+// is called for every labelled neighbour, end(s) finishes the vector.
+template <int G, int IPL, int KB, class Begin, class Use, class End>
+__device__ __forceinline__ void count_staged(const StepArgs &a, const uint32_t *__restrict__ ids, uint32_t d,
+                                             uint32_t gl, uint32_t k0, uint32_t kb, Begin &&begin, Use &&use,
+                                             End &&end) {
+    uint32_t labs[IPL][KB];
 #pragma unroll
-        for (int j = 0; j < UNROLL; j++) {
-            const uint32_t i = i0 + j * G + gl;
-            labs[j] = i < d ? clean_label(a.lin[(uint64_t)ids[i] * a.K + k]) : UNL;
+    for (int t = 0; t < IPL; t++) {
+        const uint32_t i = t * G + gl;
+        if (i < d) {
+            load_row<KB>(a, ids[i], k0, kb, labs[t]);
+        } else {
+#pragma unroll
+            for (int s = 0; s < KB; s++) labs[t][s] = UNL;
         }
+    }
 #pragma unroll
-        for (int j = 0; j < UNROLL; j++)
-            if (labs[j] != UNL) use(labs[j]);
+    for (int s = 0; s < KB; s++) {
+        if ((uint32_t)s < kb) {
+            begin(s);
+#pragma unroll
+            for (int t = 0; t < IPL; t++)
+                if (labs[t][s] != UNL) use(s, labs[t][s]);
+            for (uint32_t i = IPL * G + gl; i < d; i += G) {
+                const uint32_t lab = clean_label(a.lin[(uint64_t)ids[i] * a.K + k0 + s]);
+                if (lab != UNL) use(s, lab);
+            }
+            end(s);
+        }
     }
 }
 
-// One group of G lanes per vertex.  Dynamic LDS holds, per group, the
-// neighbour list (listcap ids, MODE_LDS) followed by 2^logT table slots;
-// for MODE_GLOBAL both live in HBM scratch, for MODE_CMS the list lives in
-// HBM scratch and the sketch in LDS.  The record is decoded once and the K
-// label vectors are counted from the list.
-template <int G, int MODE, bool CSR>
+// Neighbours staged per lane by the WAVE / BLOCK kernels: up to 8 ids per
+// lane (quarter wave <= 128, wave <= 512, workgroup <= 2048 ids) and at most
+// 32 staged labels per lane, which keeps enough waves resident to hide the
+// decode and gather latency; the rest of a list is gathered per vector.
+template <int KB> constexpr int group_ipl() { return KB >= 8 ? 4 : 8; }
+
+// One group of G lanes per vertex; dynamic LDS holds, per group, 2^logT
+// table slots followed by the neighbour list (listcap ids).  The record is
+// decoded once; label vectors are counted KB at a time from staged rows.
+template <int G, bool CSR, int KB>
 __global__ void __launch_bounds__(BLOCK) k_group(StepArgs a, uint32_t logT, uint32_t listcap) {
     constexpr int NG = BLOCK / G;
     extern __shared__ uint64_t lds[];
@@ -345,20 +412,11 @@ __global__ void __launch_bounds__(BLOCK) k_group(StepArgs a, uint32_t logT, uint
     __shared__ GroupShared<G> gsh[NG];
     const uint32_t gid = threadIdx.x / G, gl = threadIdx.x % G;
     const uint32_t T = 1u << logT;
-    uint64_t *tab;
-    uint32_t *list;
-    uint32_t *sketch = (uint32_t *)lds;
-    if (MODE == MODE_LDS) {
-        // per group: [table: T u64][list: listcap u32]
-        uint64_t *grp = lds + (uint64_t)gid * (T + (listcap + 1) / 2);
-        tab = grp;
-        list = (uint32_t *)(grp + T);
-    } else {
-        tab = a.gtab + (uint64_t)blockIdx.x * a.gcap;
-        list = a.glist + (uint64_t)blockIdx.x * a.glcap;
-    }
+    uint64_t *tab = lds + (uint64_t)gid * (T + (listcap + 1) / 2);
+    uint32_t *list = (uint32_t *)(tab + T);
     GroupShared<G> *sh = &gsh[gid];
     const uint32_t stride = gridDim.x * NG;
+    uint32_t nch = 0;
     for (uint32_t p0 = a.begin + blockIdx.x * NG; p0 < a.end; p0 += stride) {
         // all groups of a wave (or workgroup) run the same trip count
         const uint32_t p = p0 + gid;
@@ -375,67 +433,44 @@ __global__ void __launch_bounds__(BLOCK) k_group(StepArgs a, uint32_t logT, uint
         } else if (live) {
             d = group_decode<G>(a, p, v, gl, sh, list);
         }
-        for (uint32_t k = 0; k < a.K; k++) {
+        for (uint32_t k0 = 0; k0 < a.K; k0 += KB) {
+            const uint32_t kb = a.K - k0 < (uint32_t)KB ? a.K - k0 : (uint32_t)KB;
             uint64_t best = 0;
-            if (MODE == MODE_CMS) {
-                const uint32_t cells = a.cms_depth << a.cms_logw;
-                for (uint32_t i = gl; i < cells; i += G) sketch[i] = 0;
-                group_sync<G>();
-                group_count<G>(a, ids, d, gl, k, [&](uint32_t lab) {
+            count_staged<G, group_ipl<KB>(), KB>(
+                a, ids, d, gl, k0, kb,
+                [&](int) {
+                    for (uint32_t i = gl; i < T; i += G) tab[i] = EMPTY_SLOT;
+                    best = 0;
+                    group_sync<G>();
+                },
+                [&](int, uint32_t lab) {
                     const uint32_t cnt = fold_first_label<G>(lab);
                     if (cnt == 0) return;
-                    for (uint32_t r = 0; r < a.cms_depth; r++)
-                        atomicAdd(&sketch[(r << a.cms_logw) + cms_col(a, r, lab)], cnt);
-                });
-                group_sync<G>();
-                group_count<G>(a, ids, d, gl, k, [&](uint32_t lab) {
-                    uint32_t est = UNL;
-                    for (uint32_t r = 0; r < a.cms_depth; r++) {
-                        uint32_t c = sketch[(r << a.cms_logw) + cms_col(a, r, lab)];
-                        est = c < est ? c : est;
-                    }
-                    uint64_t key = ((uint64_t)est << 32) | (uint32_t)~lab;
+                    const uint32_t c = tab_add(tab, logT, lab, cnt);
+                    const uint64_t key = ((uint64_t)c << 32) | (uint32_t)~lab;
                     best = key > best ? key : best;
-                });
-            } else {
-                // HBM tables are sized per vertex: the smallest power of two >= 2d
-                const uint32_t Tv = MODE == MODE_GLOBAL ? (d ? 1u << (32 - __builtin_clz(2 * d - 1)) : 1u) : T;
-                const uint32_t lt = MODE == MODE_GLOBAL ? 31 - __builtin_clz(Tv) : logT;
-                for (uint32_t i = gl; i < Tv; i += G) tab[i] = EMPTY_SLOT;
-                if (MODE == MODE_GLOBAL) __threadfence_block();
-                group_sync<G>();
-                group_count<G>(a, ids, d, gl, k, [&](uint32_t lab) {
-                    const uint32_t cnt = fold_first_label<G>(lab);
-                    if (cnt == 0) return;
-                    uint32_t c = tab_add(tab, lt, lab, cnt);
-                    uint64_t key = ((uint64_t)c << 32) | (uint32_t)~lab;
-                    best = key > best ? key : best;
-                });
-            }
-            best = max_u64_over<(G < 64 ? G : 64)>(best);
-            if (G == BLOCK) {
-                if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = best;
-                __syncthreads();
+                },
+                [&](int s) {
+                    best = max_u64_over<(G < 64 ? G : 64)>(best);
+                    if (G == BLOCK) {
+                        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = best;
+                        __syncthreads();
 #pragma unroll
-                for (int w = 0; w < BLOCK / 64; w++) best = red[w] > best ? red[w] : best;
-            }
-            bool ch = false;
-            if (gl == 0 && live) {
-                uint64_t idx = (uint64_t)v * a.K + k;
-                uint32_t old = a.lin[idx];
-                uint32_t nl = pick(best, old);
-                a.lout[idx] = nl;
-                ch = nl != old;
-            }
-            if (G == BLOCK) {
-                if (ch) atomicAdd(a.changed, 1ull);
-            } else {
-                count_changes(ch, a.changed);
-            }
-            group_sync<G>();   // table / sketch / red reused next
+                        for (int w = 0; w < BLOCK / 64; w++) best = red[w] > best ? red[w] : best;
+                    }
+                    if (gl == 0 && live) {
+                        const uint64_t idx = (uint64_t)v * a.K + k0 + s;
+                        const uint32_t old = a.lin[idx];
+                        const uint32_t nl = pick(best, old);
+                        a.lout[idx] = nl;
+                        nch += nl != old;
+                    }
+                    group_sync<G>();   // table / red reused next
+                });
         }
         if (a.rec && live && gl == 0) a.rec[v] = (cycles() - t0) * G;
     }
+    flush_changes(nch, a.changed);
 }
 
 // --------------------------------------------------------- CGLP kernel 1
