@@ -100,6 +100,9 @@ struct HeavySet {
     DevBuf<uint64_t> tab;
     DevBuf<uint32_t> sketch;
     DevBuf<unsigned long long> result;
+    uint32_t ntasks = 0;
+    DevBuf<uint2> tasks;                     // decode tasks: (heavy vertex, first segment)
+    DevBuf<unsigned long long> cursor;       // per heavy vertex: list slots claimed by decode tasks
 };
 
 struct Bin {
@@ -331,7 +334,11 @@ void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
     // CMS: sketches of up to CMS_KB label vectors side by side in LDS
     const size_t cells = (size_t)cms_depth * cms_width;
     h.kbs = (uint32_t)std::max<size_t>(1, std::min<size_t>({(size_t)K, (size_t)CMS_KB, (size_t)(lds_max - 20480) / (cells * 4)}));
-    if (!as_csr) hipLaunchKernelGGL(k_heavy_decode, dim3(hs.nh), dim3(BLOCK), 0, stream, h);
+    if (!as_csr) {
+        hipLaunchKernelGGL(k_fill_u64, dim3((hs.nh + 255) / 256), dim3(256), 0, stream, hs.cursor.p, (uint64_t)hs.nh, 0ull);
+        hipLaunchKernelGGL(k_heavy_decode, dim3(hs.ntasks), dim3(BLOCK), 0, stream, h, hs.tasks.p, hs.ntasks,
+                           hs.cursor.p);
+    }
     if (hs.nlocal) {
         if (hs.cms)
             hipLaunchKernelGGL(k_heavy_local<1>, dim3(hs.nlocal), dim3(HBLOCK), h.kbs * cells * sizeof(uint32_t), stream,
@@ -644,7 +651,25 @@ static void build_heavy(lpa_engine *e) {
             hs->slot.upload(slot.data(), slot.size(), e->stream);
             hs->chunk_h.upload(ch.data(), ch.size(), e->stream);
             hs->chunk_b.upload(cb.data(), cb.size(), e->stream);
-            if (!hs->csr_part) hs->list.alloc(loff.back());
+            if (!hs->csr_part) {
+                hs->list.alloc(loff.back());
+                hs->cursor.alloc(hs->nh);
+                // segments per record (device metadata) -> decode tasks of <= BLOCK segments
+                DevBuf<uint4> m;
+                m.alloc(hs->nh);
+                hipLaunchKernelGGL(k_gather_u4, dim3((hs->nh + 255) / 256), dim3(256), 0, e->stream, e->meta.p,
+                                   hs->pos.p, hs->nh, m.p);
+                std::vector<uint4> hm(hs->nh);
+                HIPCHK(hipMemcpyAsync(hm.data(), m.p, hs->nh * sizeof(uint4), hipMemcpyDeviceToHost, e->stream));
+                e->sync();
+                std::vector<uint2> tk;
+                for (uint32_t i = 0; i < hs->nh; i++) {
+                    const uint32_t segs = hm[i].y + hm[i].w;
+                    for (uint32_t j0 = 0; j0 < segs; j0 += BLOCK) tk.push_back(make_uint2(i, j0));
+                }
+                hs->ntasks = (uint32_t)tk.size();
+                hs->tasks.upload(tk.data(), tk.size(), e->stream);
+            }
             if (hs->cms) {
                 hs->kb = e->K;
                 hs->sketch.alloc(((uint64_t)e->cms_depth * e->cms_width) * std::max<uint32_t>(hs->nchunked, 1) * hs->kb);

@@ -4,8 +4,8 @@
 // One workgroup per vertex cannot keep enough label gathers in flight for a
 // hub (R-MAT scale 22 has a vertex of degree 163,396; scale 27 hubs reach
 // millions), so a step over these "heavy" vertices runs as
-//   k_heavy_decode  one workgroup per heavy vertex: decode the record into a
-//                   neighbour list in HBM (long runs written by every lane),
+//   k_heavy_decode  one workgroup per run of up to 256 segments of a heavy
+//                   record: decode into the vertex's neighbour list in HBM,
 //   k_heavy_clear   clear the sketches / tables / result words of a batch of
 //                   label vectors [k0, k0+kb) (their memory is reused by the
 //                   next batch when K is large),
@@ -64,14 +64,89 @@ __device__ __forceinline__ uint32_t heavy_deg(const HeavyArgs &h, uint32_t i) {
     return (uint32_t)(h.loff[i + 1] - h.loff[i]);
 }
 
-__global__ void __launch_bounds__(BLOCK) k_heavy_decode(HeavyArgs h) {
+// One workgroup per decode task: up to BLOCK consecutive segments (one per
+// lane; intervals first, then residuals) of heavy vertex tasks[t].x starting
+// at segment tasks[t].y, so a hub's record is decoded by many workgroups at
+// once.  The order of a list is irrelevant to the counts, so a task takes
+// its output range from the vertex's cursor (zero before the step) with one
+// atomic: lanes first count their segments' ids (pass A), then decode again
+// into their part of the range (pass B); runs of LONG_RUN or more ids are
+// written by the whole workgroup.  A task's ids are assembled in LDS and
+// copied out with coalesced stores (one lane per id), unless they exceed
+// the staging buffer.
+constexpr uint32_t DECODE_STAGE = 8192;
+
+__global__ void __launch_bounds__(BLOCK) k_heavy_decode(HeavyArgs h, const uint2 *__restrict__ tasks,
+                                                        uint32_t ntasks, unsigned long long *__restrict__ cursor) {
     __shared__ GroupShared<BLOCK> sh;
-    const uint32_t i = blockIdx.x;
-    if (i >= h.nh || h.csr) return;
+    __shared__ uint64_t base;
+    __shared__ uint32_t stage[DECODE_STAGE];
+    const uint32_t t = blockIdx.x;
+    if (t >= ntasks || h.csr) return;
     const uint64_t t0 = h.s.rec ? cycles() : 0;
+    const uint32_t i = tasks[t].x, j = tasks[t].y + threadIdx.x;
     const uint32_t p = h.pos[i];
-    group_decode<BLOCK>(h.s, p, h.s.vid[p], threadIdx.x, &sh, h.list + h.loff[i]);
-    if (h.s.rec && threadIdx.x == 0) atomicAdd(&h.s.rec[h.s.vid[p]], (cycles() - t0) * BLOCK);
+    const uint32_t v = h.s.vid[p];
+    const uint64_t start = h.s.offs[p];
+    const uint4 m = h.s.meta[p];
+    const uint32_t ni = m.y, ns = m.y + m.w;
+    if (threadIdx.x == 0) {
+        sh.cnt = 0;
+        sh.qn = 0;
+    }
+    __syncthreads();
+    BitReader r;
+    uint32_t mine = 0;
+    if (j < ns) {
+        if (j < ni) {
+            r.seek(h.s.words, start + m.x + (uint64_t)SEG_BITS * j);
+            BitReader q = r;
+            read_interval_segment(q, v, [&](uint32_t, uint32_t len) { mine += len; });
+        } else {
+            r.seek(h.s.words, start + m.z + (uint64_t)SEG_BITS * (j - ni));
+            BitReader q = r;
+            mine = q.gamma();
+        }
+    }
+    uint32_t o = mine ? atomicAdd(&sh.cnt, mine) : 0;
+    __syncthreads();
+    const uint32_t total = sh.cnt;
+    if (threadIdx.x == 0) base = atomicAdd(&cursor[i], (unsigned long long)total);
+    __syncthreads();
+    uint32_t *out = h.list + h.loff[i] + base;
+    const bool staged = total <= DECODE_STAGE;
+    uint32_t *dst = staged ? stage : out;
+    if (j < ns) {
+        if (j < ni) {
+            read_interval_segment(r, v, [&](uint32_t l, uint32_t len) {
+                if (len >= LONG_RUN) {
+                    const uint32_t q = atomicAdd(&sh.qn, 1u);
+                    if (q < (uint32_t)GroupShared<BLOCK>::QCAP) {
+                        sh.qleft[q] = l;
+                        sh.qlen[q] = len;
+                        sh.qpos[q] = o;
+                        o += len;
+                        return;
+                    }
+                }
+                for (uint32_t x = 0; x < len; x++) dst[o + x] = l + x;
+                o += len;
+            });
+        } else {
+            read_residual_segment(r, v, [&](uint32_t u) { dst[o++] = u; });
+        }
+    }
+    __syncthreads();
+    const uint32_t nq = sh.qn < (uint32_t)GroupShared<BLOCK>::QCAP ? sh.qn : (uint32_t)GroupShared<BLOCK>::QCAP;
+    for (uint32_t q = 0; q < nq; q++) {
+        const uint32_t l = sh.qleft[q], len = sh.qlen[q], qo = sh.qpos[q];
+        for (uint32_t x = threadIdx.x; x < len; x += BLOCK) dst[qo + x] = l + x;
+    }
+    if (staged) {
+        __syncthreads();
+        for (uint32_t x = threadIdx.x; x < total; x += BLOCK) out[x] = stage[x];
+    }
+    if (h.s.rec && threadIdx.x == 0) atomicAdd(&h.s.rec[v], (cycles() - t0) * BLOCK);
 }
 
 // chunked[] lists the heavy-vertex indices handled by the chunk kernels.

@@ -169,6 +169,12 @@ __global__ void k_pack64(const uint32_t *__restrict__ w32, uint64_t n32, uint64_
     w64[i] = (hi << 32) | lo;
 }
 
+__global__ void k_gather_u4(const uint4 *__restrict__ src, const uint32_t *__restrict__ idx, uint32_t n,
+                            uint4 *__restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = src[idx[i]];
+}
+
 __global__ void k_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
