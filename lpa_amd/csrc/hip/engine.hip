@@ -331,34 +331,46 @@ void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
     h.chunk_b = hs.chunk_b.p;
     h.cms = hs.cms ? 1 : 0;
     h.slot = hs.slot.p;
-    // CMS: sketches of up to CMS_KB label vectors side by side in LDS
+    // CMS: sketches of up to KB label vectors side by side in LDS
+    const bool one = K == 1;
     const size_t cells = (size_t)cms_depth * cms_width;
-    h.kbs = (uint32_t)std::max<size_t>(1, std::min<size_t>({(size_t)K, (size_t)CMS_KB, (size_t)(lds_max - 20480) / (cells * 4)}));
+    h.kbs = (uint32_t)std::max<size_t>(
+        1, std::min<size_t>({(size_t)K, (size_t)(one ? 1 : HEAVY_KB_MAX), (size_t)(lds_max - 20480) / (cells * 4)}));
     if (!as_csr) {
         hipLaunchKernelGGL(k_fill_u64, dim3((hs.nh + 255) / 256), dim3(256), 0, stream, hs.cursor.p, (uint64_t)hs.nh, 0ull);
         hipLaunchKernelGGL(k_heavy_decode, dim3(hs.ntasks), dim3(BLOCK), 0, stream, h, hs.tasks.p, hs.ntasks,
                            hs.cursor.p);
     }
     if (hs.nlocal) {
-        if (hs.cms)
-            hipLaunchKernelGGL(k_heavy_local<1>, dim3(hs.nlocal), dim3(HBLOCK), h.kbs * cells * sizeof(uint32_t), stream,
-                               h, hs.local.p, hs.nlocal);
-        else
-            hipLaunchKernelGGL(k_heavy_local<0>, dim3(hs.nlocal), dim3(HBLOCK), 2 * CHUNK * sizeof(uint64_t), stream,
-                               h, hs.local.p, hs.nlocal);
+        const dim3 g(hs.nlocal), b(HBLOCK);
+        const size_t sk = h.kbs * cells * sizeof(uint32_t), tb = 2 * CHUNK * sizeof(uint64_t);
+        if (hs.cms) {
+            if (one) hipLaunchKernelGGL((k_heavy_local<1, 1>), g, b, sk, stream, h, hs.local.p, hs.nlocal);
+            else hipLaunchKernelGGL((k_heavy_local<1, HEAVY_KB_MAX>), g, b, sk, stream, h, hs.local.p, hs.nlocal);
+        } else {
+            if (one) hipLaunchKernelGGL((k_heavy_local<0, 1>), g, b, tb, stream, h, hs.local.p, hs.nlocal);
+            else hipLaunchKernelGGL((k_heavy_local<0, HEAVY_KB_MAX>), g, b, tb, stream, h, hs.local.p, hs.nlocal);
+        }
     }
     if (hs.nchunked == 0) return;
     for (uint32_t k0 = 0; k0 < K; k0 += hs.kb) {
         h.k0 = k0;
         h.kb = std::min(hs.kb, K - k0);
         hipLaunchKernelGGL(k_heavy_clear, dim3(hs.nchunked), dim3(BLOCK), 0, stream, h, hs.chunked.p, hs.nchunked);
+        const dim3 g(hs.nchunks), b(HBLOCK);
         if (hs.cms) {
             const size_t sk = std::min(h.kbs, h.kb) * cells * sizeof(uint32_t);
-            hipLaunchKernelGGL(k_heavy_chunks<1>, dim3(hs.nchunks), dim3(HBLOCK), sk, stream, h);
-            hipLaunchKernelGGL(k_heavy_chunks<2>, dim3(hs.nchunks), dim3(HBLOCK), 0, stream, h);
+            if (one) {
+                hipLaunchKernelGGL((k_heavy_chunks<1, 1>), g, b, sk, stream, h);
+                hipLaunchKernelGGL((k_heavy_chunks<2, 1>), g, b, 0, stream, h);
+            } else {
+                hipLaunchKernelGGL((k_heavy_chunks<1, HEAVY_KB_MAX>), g, b, sk, stream, h);
+                hipLaunchKernelGGL((k_heavy_chunks<2, HEAVY_KB_MAX>), g, b, 0, stream, h);
+            }
         } else {
-            hipLaunchKernelGGL(k_heavy_chunks<0>, dim3(hs.nchunks), dim3(HBLOCK), 2 * CHUNK * sizeof(uint64_t),
-                               stream, h);
+            const size_t tb = 2 * CHUNK * sizeof(uint64_t);
+            if (one) hipLaunchKernelGGL((k_heavy_chunks<0, 1>), g, b, tb, stream, h);
+            else hipLaunchKernelGGL((k_heavy_chunks<0, HEAVY_KB_MAX>), g, b, tb, stream, h);
         }
         const uint64_t fin = (uint64_t)hs.nchunked * h.kb;
         hipLaunchKernelGGL(k_heavy_finish, dim3((unsigned)((fin + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, stream, h,
@@ -436,10 +448,11 @@ int lpa_engine_create(int device, lpa_engine **out) {
             allow_group_lds<16>(dyn);
             allow_group_lds<64>(dyn);
             allow_group_lds<BLOCK>(dyn);
-            (void)hipFuncSetAttribute((const void *)k_heavy_chunks<0>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
-            (void)hipFuncSetAttribute((const void *)k_heavy_local<0>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
-            (void)hipFuncSetAttribute((const void *)k_heavy_local<1>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
-            (void)hipFuncSetAttribute((const void *)k_heavy_chunks<1>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+            const void *heavy_fns[] = {(const void *)k_heavy_chunks<0, 1>, (const void *)k_heavy_chunks<0, HEAVY_KB_MAX>,
+                                       (const void *)k_heavy_chunks<1, 1>, (const void *)k_heavy_chunks<1, HEAVY_KB_MAX>,
+                                       (const void *)k_heavy_local<0, 1>, (const void *)k_heavy_local<0, HEAVY_KB_MAX>,
+                                       (const void *)k_heavy_local<1, 1>, (const void *)k_heavy_local<1, HEAVY_KB_MAX>};
+            for (const void *f : heavy_fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
             (void)hipGetLastError();
         } catch (...) {
             delete e;

@@ -176,36 +176,37 @@ __device__ __forceinline__ uint64_t block_max(uint64_t x, uint64_t *red) {
 }
 
 // Exact counting stages the label rows of up to CHUNK neighbours per
-// workgroup in registers (HEAVY_IPL per lane), HEAVY_KB vectors at a time.
+// workgroup in registers (HEAVY_IPL per lane), KB vectors at a time.
 constexpr int HEAVY_IPL = CHUNK / HBLOCK;
-constexpr int HEAVY_KB = 8;
 // Sketch passes walk the list in rounds of CMS_R neighbours per lane and
-// serve up to CMS_KB label vectors (kbs sketches side by side) per round.
+// serve up to KB label vectors (kbs sketches side by side) per round.
 constexpr int CMS_R = 4;
-constexpr int CMS_KB = 8;
+// Label vectors per staged batch of the heavy kernels (template KB): 1 for
+// a single vector, else 8.
+constexpr int HEAVY_KB_MAX = 8;
 
-// One sketch pass over ids[0..n) for vectors k0 .. k0+kb-1 (kb <= CMS_KB;
+// One sketch pass over ids[0..n) for vectors k0 .. k0+kb-1 (kb <= KB;
 // sketch s at sk + s*cells).  ADD: add every label to its sketch (LDS,
 // pre-aggregated per wave); else: estimate every label from its sketch and
 // keep the best (count << 32 | ~label) per vector in best[s].
-template <bool ADD>
+template <bool ADD, int KB>
 __device__ __forceinline__ void cms_rounds(const StepArgs &a, const uint32_t *__restrict__ ids, uint32_t n,
                                            uint32_t k0, uint32_t kb, uint32_t *sk, uint32_t cells,
-                                           uint64_t (&best)[CMS_KB]) {
+                                           uint64_t (&best)[KB]) {
     for (uint32_t i0 = 0; i0 < n; i0 += CMS_R * HBLOCK) {
-        uint32_t labs[CMS_R][CMS_KB];
+        uint32_t labs[CMS_R][KB];
 #pragma unroll
         for (int t = 0; t < CMS_R; t++) {
             const uint32_t i = i0 + t * HBLOCK + threadIdx.x;
             if (i < n) {
-                load_row<CMS_KB>(a, ids[i], k0, kb, labs[t]);
+                load_row<KB>(a, ids[i], k0, kb, labs[t]);
             } else {
 #pragma unroll
-                for (int q = 0; q < CMS_KB; q++) labs[t][q] = UNL;
+                for (int q = 0; q < KB; q++) labs[t][q] = UNL;
             }
         }
 #pragma unroll
-        for (int q = 0; q < CMS_KB; q++) {
+        for (int q = 0; q < KB; q++) {
             if ((uint32_t)q >= kb) continue;
             uint32_t *skq = sk + (uint64_t)q * cells;
 #pragma unroll
@@ -238,7 +239,7 @@ __device__ __forceinline__ void cms_rounds(const StepArgs &a, const uint32_t *__
 // into the vertex's HBM sketch or table, so a hub whose neighbours share a
 // few labels does not funnel every arc into the same HBM atomic.  Dynamic
 // LDS: kbs local sketches (pass 1) or the local table (pass 0).
-template <int PASS>
+template <int PASS, int KB>
 __global__ void __launch_bounds__(HBLOCK) k_heavy_chunks(HeavyArgs h) {
     extern __shared__ uint64_t lds[];
     __shared__ uint64_t red[HBLOCK / 64];
@@ -257,10 +258,10 @@ __global__ void __launch_bounds__(HBLOCK) k_heavy_chunks(HeavyArgs h) {
         const uint32_t logT = 63 - __builtin_clzll(T);
         constexpr uint32_t LT = 2 * CHUNK, LOG_LT = 31 - __builtin_clz(LT);
         uint64_t *ltab = lds;
-        for (uint32_t kq = 0; kq < h.kb; kq += HEAVY_KB) {
-            const uint32_t kb = h.kb - kq < (uint32_t)HEAVY_KB ? h.kb - kq : (uint32_t)HEAVY_KB;
+        for (uint32_t kq = 0; kq < h.kb; kq += KB) {
+            const uint32_t kb = h.kb - kq < (uint32_t)KB ? h.kb - kq : (uint32_t)KB;
             uint64_t best = 0;
-            count_staged<HBLOCK, HEAVY_IPL, HEAVY_KB>(
+            count_staged<HBLOCK, HEAVY_IPL, KB>(
                 a, ids, n, threadIdx.x, h.k0 + kq, kb,
                 [&](int) {
                     for (uint32_t x = threadIdx.x; x < LT; x += HBLOCK) ltab[x] = EMPTY_SLOT;
@@ -292,20 +293,20 @@ __global__ void __launch_bounds__(HBLOCK) k_heavy_chunks(HeavyArgs h) {
         for (uint32_t kq = 0; kq < h.kb; kq += h.kbs) {
             const uint32_t kb = h.kb - kq < h.kbs ? h.kb - kq : h.kbs;
             uint32_t *hsk = h.sketch + ((uint64_t)sl * h.kb + kq) * cells;
-            uint64_t best[CMS_KB] = {};
+            uint64_t best[KB] = {};
             if (PASS == 1) {
                 uint32_t *lsk = (uint32_t *)lds;
                 for (uint32_t x = threadIdx.x; x < kb * cells; x += HBLOCK) lsk[x] = 0;
                 __syncthreads();
-                cms_rounds<true>(a, ids, n, h.k0 + kq, kb, lsk, cells, best);
+                cms_rounds<true, KB>(a, ids, n, h.k0 + kq, kb, lsk, cells, best);
                 __syncthreads();
                 for (uint32_t x = threadIdx.x; x < kb * cells; x += HBLOCK)
                     if (lsk[x]) atomicAdd(&hsk[x], lsk[x]);
                 __syncthreads();
             } else {
-                cms_rounds<false>(a, ids, n, h.k0 + kq, kb, hsk, cells, best);
+                cms_rounds<false, KB>(a, ids, n, h.k0 + kq, kb, hsk, cells, best);
 #pragma unroll
-                for (int q = 0; q < CMS_KB; q++) {
+                for (int q = 0; q < KB; q++) {
                     if ((uint32_t)q >= kb) continue;
                     const uint64_t m = block_max(best[q], red);
                     if (threadIdx.x == 0 && m)
@@ -326,7 +327,7 @@ __global__ void __launch_bounds__(HBLOCK) k_heavy_chunks(HeavyArgs h) {
 // list -- both CMS passes, or the exact table -- and writes the labels
 // itself, with no HBM table, sketch or merge.  local[] lists the
 // heavy-vertex indices.
-template <int CMS>
+template <int CMS, int KB>
 __global__ void __launch_bounds__(HBLOCK) k_heavy_local(HeavyArgs h, const uint32_t *__restrict__ local,
                                                        uint32_t nlocal) {
     extern __shared__ uint64_t lds[];
@@ -354,12 +355,12 @@ __global__ void __launch_bounds__(HBLOCK) k_heavy_local(HeavyArgs h, const uint3
             const uint32_t kb = a.K - k0 < h.kbs ? a.K - k0 : h.kbs;
             for (uint32_t x = threadIdx.x; x < kb * cells; x += HBLOCK) sk[x] = 0;
             __syncthreads();
-            uint64_t best[CMS_KB] = {};
-            cms_rounds<true>(a, ids, d, k0, kb, sk, cells, best);
+            uint64_t best[KB] = {};
+            cms_rounds<true, KB>(a, ids, d, k0, kb, sk, cells, best);
             __syncthreads();
-            cms_rounds<false>(a, ids, d, k0, kb, sk, cells, best);
+            cms_rounds<false, KB>(a, ids, d, k0, kb, sk, cells, best);
 #pragma unroll
-            for (int q = 0; q < CMS_KB; q++) {
+            for (int q = 0; q < KB; q++) {
                 if ((uint32_t)q >= kb) continue;
                 const uint64_t m = block_max(best[q], red);
                 if (threadIdx.x == 0) finish(k0 + q, m);
@@ -369,10 +370,10 @@ __global__ void __launch_bounds__(HBLOCK) k_heavy_local(HeavyArgs h, const uint3
         const uint32_t T = d ? 1u << (32 - __builtin_clz(2 * d - 1)) : 1u;
         const uint32_t logT = 31 - __builtin_clz(T);
         uint64_t *tab = lds;
-        for (uint32_t k0 = 0; k0 < a.K; k0 += HEAVY_KB) {
-            const uint32_t kb = a.K - k0 < (uint32_t)HEAVY_KB ? a.K - k0 : (uint32_t)HEAVY_KB;
+        for (uint32_t k0 = 0; k0 < a.K; k0 += KB) {
+            const uint32_t kb = a.K - k0 < (uint32_t)KB ? a.K - k0 : (uint32_t)KB;
             uint64_t best = 0;
-            count_staged<HBLOCK, HEAVY_IPL, HEAVY_KB>(
+            count_staged<HBLOCK, HEAVY_IPL, KB>(
                 a, ids, d, threadIdx.x, k0, kb,
                 [&](int) {
                     for (uint32_t x = threadIdx.x; x < T; x += HBLOCK) tab[x] = EMPTY_SLOT;
