@@ -93,8 +93,9 @@ def timed(e, steps, warmup, method="nova", labels=None):
     return ms, changed, e.kernel_times()
 
 
-def train_model(L, cgr_main, w):
-    """cost model trained on work recorded on the lattice and on this graph"""
+def train_model(L, cgr_main, w, K):
+    """cost model trained on work recorded (K label vectors) on the lattice
+    and on this graph"""
     samples = []
     lat = L.gen_lattice(1024, 0.10, 0.005, seed=0).encode(0, 128)
     r22 = cgr_main
@@ -104,7 +105,7 @@ def train_model(L, cgr_main, w):
     for cgr, cms in ((lat, {}), (r22, WORKLOADS["rmat22"]["cms"])):
         e = L.Engine(0)
         e.upload(cgr)
-        e.config(K=1, **cms)
+        e.config(K=K, **cms)
         e.schedule_degree()
         e.labels_init()
         e.nova_iterate()
@@ -123,6 +124,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-cglp", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the headline phase (for profiling): no K=1, two-kernel or degree-binned K runs")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="finish with three plain device copies of 256 MiB (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus})",
@@ -143,26 +148,29 @@ def main():
     semi = bool(w.get("semi"))
     lab1 = semi_labels(g.n, 1, w["seed"]) if semi else None
     labK = semi_labels(g.n, K, w["seed"]) if semi else None
-    # degree-binned plain kernel, K = 1
-    e.config(K=1, semi=semi, **w["cms"])
-    e.schedule_degree()
-    ms, ch1, kt1 = timed(e, args.steps, args.warmup, labels=lab1)
-    res["degree_k1"] = dict(edges_per_s=m / (float(np.mean(ms)) / 1e3), ms_per_step=float(np.mean(ms)),
-                            kernel_ms=kt1, changed=[int(x) for x in ch1])
-    log(f"degree K=1: {np.mean(ms):.3f} ms/step")
-    if not args.no_cglp:
-        ms2, _, _ = timed(e, max(2, args.steps // 2), 1, "cglp", labels=lab1)
-        res["cglp_k1"] = dict(edges_per_s=m / (float(np.mean(ms2)) / 1e3), ms_per_step=float(np.mean(ms2)))
-        log(f"cglp K=1: {np.mean(ms2):.3f} ms/step")
-    # degree-binned, K vectors
-    e.config(K=K, semi=semi, **w["cms"])
-    e.schedule_degree()
-    msk, chk, ktk = timed(e, args.steps, args.warmup, labels=labK)
-    res[f"degree_k{K}"] = dict(edges_per_s=K * m / (float(np.mean(msk)) / 1e3), ms_per_step=float(np.mean(msk)))
-    log(f"degree K={K}: {np.mean(msk):.3f} ms/step")
-    head_ms, head_kt, schedule = msk, ktk, "degree"
+    if not args.headline_only:
+        # degree-binned plain kernel, K = 1
+        e.config(K=1, semi=semi, **w["cms"])
+        e.schedule_degree()
+        ms, ch1, kt1 = timed(e, args.steps, args.warmup, labels=lab1)
+        res["degree_k1"] = dict(edges_per_s=m / (float(np.mean(ms)) / 1e3), ms_per_step=float(np.mean(ms)),
+                                kernel_ms=kt1, changed=[int(x) for x in ch1])
+        log(f"degree K=1: {np.mean(ms):.3f} ms/step")
+        if not args.no_cglp:
+            ms2, _, _ = timed(e, max(2, args.steps // 2), 1, "cglp", labels=lab1)
+            res["cglp_k1"] = dict(edges_per_s=m / (float(np.mean(ms2)) / 1e3), ms_per_step=float(np.mean(ms2)))
+            log(f"cglp K=1: {np.mean(ms2):.3f} ms/step")
+    head_ms = None
+    if not args.headline_only or args.no_learned:
+        # degree-binned, K vectors
+        e.config(K=K, semi=semi, **w["cms"])
+        e.schedule_degree()
+        msk, chk, ktk = timed(e, args.steps, args.warmup, labels=labK)
+        res[f"degree_k{K}"] = dict(edges_per_s=K * m / (float(np.mean(msk)) / 1e3), ms_per_step=float(np.mean(msk)))
+        log(f"degree K={K}: {np.mean(msk):.3f} ms/step")
+        head_ms, head_kt, schedule = msk, ktk, "degree"
     if not args.no_learned:
-        model, losses = train_model(L, cgr, w)
+        model, losses = train_model(L, cgr, w, K)
         e.config(K=K, semi=semi, **w["cms"])
         e.schedule_learned(model)
         msl, chl, ktl = timed(e, args.steps, args.warmup, labels=labK)
@@ -171,6 +179,9 @@ def main():
                                     bins=e.counts())
         log(f"learned K={K}: {np.mean(msl):.3f} ms/step")
         head_ms, head_kt, schedule = msl, ktl, "learned"
+    if args.copy_calibration:
+        cms_ = e.calibration_copy(256 << 20, 3)
+        log(f"copy calibration: 3 copies of 256 MiB, {cms_:.3f} ms each ({2 * 256 * 1.048576 / cms_:.0f} GB/s)")
     stats = e.stats()
     ms_per_step = float(np.mean(head_ms))
     value = K * m * args.steps / (float(np.sum(head_ms)) / 1e3)
@@ -204,10 +215,29 @@ def main():
         "device": e.info(),
         **res,
     }
-    line["speedup_vs_degree_k1"] = value / res["degree_k1"]["edges_per_s"]
+    tr = measured_traffic(args.workload, K, dom)
+    if tr:
+        line["roofline"]["traffic"] = tr["bytes"]
+        line["roofline"]["traffic_source"] = tr["source"]
+    if "degree_k1" in res:
+        line["speedup_vs_degree_k1"] = value / res["degree_k1"]["edges_per_s"]
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline_run(cgr, g.degrees(), w)
     print(json.dumps(line))
+
+
+def measured_traffic(workload, K, fam):
+    """HBM bytes per step of a kernel family from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (copy-calibrated)"""
+    path = os.path.join(ROOT, "profiles", f"{workload}_k{K}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    t = d.get("traffic_per_iter", {}).get(fam)
+    if not t:
+        return None
+    return {"bytes": t["fetch_bytes"] + t["write_bytes"], "source": os.path.relpath(path, ROOT)}
 
 
 def cpu_baseline_run(cgr, deg, w, budget_s=20.0):

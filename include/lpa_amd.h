@@ -161,8 +161,10 @@ typedef struct lpa_mlp lpa_mlp;
 int lpa_model_features(lpa_engine *e, float *x);
 /* Runs one synchronous step with every vertex the schedule allows forced
  * into kernel family `family` (0..2) and the recorder on: cost[v] receives
- * the lane-cycles spent on v, measured[v] = 1 when v ran in that family.
- * Labels and the previous schedule are restored afterwards. */
+ * v's share of the device time in picoseconds (each bin's HIP-event time
+ * split among its vertices in proportion to the lane-cycles spent on
+ * them), measured[v] = 1 when v ran in that family.  Labels and the
+ * previous schedule are restored afterwards. */
 int lpa_record_family(lpa_engine *e, int family, uint64_t *cost, uint8_t *measured);
 
 int lpa_mlp_create(int device, uint64_t seed, lpa_mlp **out);
@@ -198,6 +200,12 @@ int lpa_autotune(lpa_engine *e, int iters, lpa_sched_params *best, float *best_m
 /* Average time (ms) per step of each kernel family in the last eager timed
  * run (HIP events on the engine stream). */
 int lpa_last_kernel_times(lpa_engine *e, float *ms_per_kind /* LPA_NBIN_KINDS */);
+/* Measurement helper, no counterpart in the reference: copies `bytes`
+ * (a multiple of 16) between two fresh device buffers `reps` times with a
+ * plain 16-byte-per-lane copy kernel (k_calibration_copy), so hardware
+ * counters of the step kernels can be scaled by what they read for a copy
+ * of known size; *ms = average kernel time. */
+int lpa_calibration_copy(lpa_engine *e, uint64_t bytes, int reps, float *ms);
 
 #ifdef __cplusplus
 }

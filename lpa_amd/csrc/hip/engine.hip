@@ -233,6 +233,7 @@ struct lpa_engine {
     void launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr);
     void step(bool cglp, hipEvent_t *kind_events);
     void run_heavy(bool cglp, hipEvent_t *kind_events, bool *started);
+    void step_bin_times(std::vector<float> &bin_ms, std::vector<float> &heavy_ms);
 };
 
 // ---------------------------------------------------------------- launches
@@ -257,31 +258,35 @@ static int wave_groups(uint32_t logT, uint32_t quarter_max = 128) {
 // label vectors staged per row gather by the group kernels
 static int group_kb(uint32_t K) { return K >= 5 ? 8 : K >= 3 ? 4 : K == 2 ? 2 : 1; }
 
-template <int G, bool CSR, int KB> static const void *group_fn() { return (const void *)k_group<G, CSR, KB>; }
+// instance of the group kernel for a batch size and staged ids per lane
+template <int G, bool CSR> static const void *group_fn(int kb, uint32_t ipl) {
+    switch (kb) {
+    case 1: return (const void *)k_group<G, CSR, 1, 8>;
+    case 2: return (const void *)k_group<G, CSR, 2, 8>;
+    case 4:
+        return ipl <= 2 ? (const void *)k_group<G, CSR, 4, 2>
+             : ipl <= 4 ? (const void *)k_group<G, CSR, 4, 4> : (const void *)k_group<G, CSR, 4, 8>;
+    default:
+        return ipl <= 2 ? (const void *)k_group<G, CSR, 8, 2>
+             : ipl <= 4 ? (const void *)k_group<G, CSR, 8, 4> : (const void *)k_group<G, CSR, 8, 8>;
+    }
+}
 
 template <int G>
 static void launch_group(bool as_csr, uint32_t K, dim3 grid, size_t lds, hipStream_t s, const StepArgs &a,
                          uint32_t logT, uint32_t cap) {
-    const void *fn;
-    switch (group_kb(K) * 2 + (as_csr ? 1 : 0)) {
-    case 2: fn = group_fn<G, false, 1>(); break;
-    case 3: fn = group_fn<G, true, 1>(); break;
-    case 4: fn = group_fn<G, false, 2>(); break;
-    case 5: fn = group_fn<G, true, 2>(); break;
-    case 8: fn = group_fn<G, false, 4>(); break;
-    case 9: fn = group_fn<G, true, 4>(); break;
-    case 16: fn = group_fn<G, false, 8>(); break;
-    default: fn = group_fn<G, true, 8>(); break;
-    }
+    const uint32_t ipl = (cap + G - 1) / G;
+    const void *fn = as_csr ? group_fn<G, true>(group_kb(K), ipl) : group_fn<G, false>(group_kb(K), ipl);
     StepArgs args = a;
     void *params[] = {&args, &logT, &cap};
     (void)hipLaunchKernel(fn, grid, dim3(BLOCK), params, lds, s);
 }
 
 template <int G> static void allow_group_lds(int dyn) {
-    const void *fns[] = {group_fn<G, false, 1>(), group_fn<G, true, 1>(), group_fn<G, false, 2>(), group_fn<G, true, 2>(),
-                         group_fn<G, false, 4>(), group_fn<G, true, 4>(), group_fn<G, false, 8>(), group_fn<G, true, 8>()};
-    for (const void *f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+    for (int kb : {1, 2, 4, 8})
+        for (uint32_t ipl : {2u, 4u, 8u})
+            for (const void *f : {group_fn<G, false>(kb, ipl), group_fn<G, true>(kb, ipl)})
+                (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
 }
 
 void lpa_engine::launch_bin(const Bin &b, StepArgs a, bool as_csr) {
@@ -338,8 +343,12 @@ void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
         1, std::min<size_t>({(size_t)K, (size_t)(one ? 1 : HEAVY_KB_MAX), (size_t)(lds_max - 20480) / (cells * 4)}));
     if (!as_csr) {
         hipLaunchKernelGGL(k_fill_u64, dim3((hs.nh + 255) / 256), dim3(256), 0, stream, hs.cursor.p, (uint64_t)hs.nh, 0ull);
-        hipLaunchKernelGGL(k_heavy_decode, dim3(hs.ntasks), dim3(BLOCK), 0, stream, h, hs.tasks.p, hs.ntasks,
-                           hs.cursor.p);
+        if (hs.cms)
+            hipLaunchKernelGGL(k_heavy_decode<1>, dim3(hs.ntasks), dim3(BLOCK), 0, stream, h, hs.tasks.p, hs.ntasks,
+                               hs.cursor.p);
+        else
+            hipLaunchKernelGGL(k_heavy_decode<0>, dim3(hs.ntasks), dim3(BLOCK), 0, stream, h, hs.tasks.p, hs.ntasks,
+                               hs.cursor.p);
     }
     if (hs.nlocal) {
         const dim3 g(hs.nlocal), b(HBLOCK);
@@ -356,7 +365,8 @@ void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
     for (uint32_t k0 = 0; k0 < K; k0 += hs.kb) {
         h.k0 = k0;
         h.kb = std::min(hs.kb, K - k0);
-        hipLaunchKernelGGL(k_heavy_clear, dim3(hs.nchunked), dim3(BLOCK), 0, stream, h, hs.chunked.p, hs.nchunked);
+        if (hs.cms) hipLaunchKernelGGL(k_heavy_clear<1>, dim3(hs.nchunked), dim3(BLOCK), 0, stream, h, hs.chunked.p, hs.nchunked);
+        else hipLaunchKernelGGL(k_heavy_clear<0>, dim3(hs.nchunked), dim3(BLOCK), 0, stream, h, hs.chunked.p, hs.nchunked);
         const dim3 g(hs.nchunks), b(HBLOCK);
         if (hs.cms) {
             const size_t sk = std::min(h.kbs, h.kb) * cells * sizeof(uint32_t);
@@ -373,8 +383,9 @@ void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
             else hipLaunchKernelGGL((k_heavy_chunks<0, HEAVY_KB_MAX>), g, b, tb, stream, h);
         }
         const uint64_t fin = (uint64_t)hs.nchunked * h.kb;
-        hipLaunchKernelGGL(k_heavy_finish, dim3((unsigned)((fin + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, stream, h,
-                           hs.chunked.p, hs.nchunked);
+        const dim3 fg((unsigned)((fin + BLOCK - 1) / BLOCK));
+        if (hs.cms) hipLaunchKernelGGL(k_heavy_finish<1>, fg, dim3(BLOCK), 0, stream, h, hs.chunked.p, hs.nchunked);
+        else hipLaunchKernelGGL(k_heavy_finish<0>, fg, dim3(BLOCK), 0, stream, h, hs.chunked.p, hs.nchunked);
     }
 }
 
@@ -411,6 +422,50 @@ void lpa_engine::run_heavy(bool cglp, hipEvent_t *kind_events, bool *started) {
         launch_heavy(hs, a, hs.csr_part || cglp);
         if (kind_events) HIPCHK(hipEventRecord(kind_events[2 * kind + 1], stream));
     }
+}
+
+// One fused step like step(false, ...), timing every bin launch and every
+// heavy set with HIP events (ms; 0 where nothing ran).
+void lpa_engine::step_bin_times(std::vector<float> &bin_ms, std::vector<float> &heavy_ms) {
+    const size_t nb = bins.size(), nhs = heavy.size();
+    bin_ms.assign(nb, 0.0f);
+    heavy_ms.assign(nhs, 0.0f);
+    std::vector<hipEvent_t> ev(2 * (nb + nhs), nullptr);
+    std::vector<char> used(nb + nhs, 0);
+    try {
+        for (auto &x : ev) HIPCHK(hipEventCreate(&x));
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, stream, counter.p, (uint64_t)1, 0ull);
+        const StepArgs a = base_args(false);
+        for (size_t i = 0; i < nb; i++) {
+            const Bin &b = bins[i];
+            if (b.kind == LPA_BIN_SKIP || b.kind == LPA_BIN_CMS || b.kind == LPA_BIN_GLOBAL || b.end == b.begin) continue;
+            HIPCHK(hipEventRecord(ev[2 * i], stream));
+            launch_bin(b, a, b.csr);
+            HIPCHK(hipEventRecord(ev[2 * i + 1], stream));
+            used[i] = 1;
+        }
+        for (size_t j = 0; j < nhs; j++) {
+            HeavySet &hs = *heavy[j];
+            if (hs.nh == 0) continue;
+            HIPCHK(hipEventRecord(ev[2 * (nb + j)], stream));
+            launch_heavy(hs, base_args(false), hs.csr_part);
+            HIPCHK(hipEventRecord(ev[2 * (nb + j) + 1], stream));
+            used[nb + j] = 1;
+        }
+        HIPCHK(hipGetLastError());
+        sync();
+        for (size_t i = 0; i < nb + nhs; i++) {
+            if (!used[i]) continue;
+            float t = 0;
+            HIPCHK(hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]));
+            (i < nb ? bin_ms[i] : heavy_ms[i - nb]) = t;
+        }
+    } catch (...) {
+        for (auto &x : ev)
+            if (x) (void)hipEventDestroy(x);
+        throw;
+    }
+    for (auto &x : ev) (void)hipEventDestroy(x);
 }
 
 // ------------------------------------------------------------------ C ABI
@@ -1193,11 +1248,41 @@ int lpa_record_family(lpa_engine *e, int family, uint64_t *cost, uint8_t *measur
             rec.alloc(e->n);
             HIPCHK(hipMemsetAsync(rec.p, 0, e->n * 8, e->stream));
             e->rec = rec.p;
-            e->step(false, nullptr);
+            std::vector<float> bin_ms, heavy_ms;
+            e->step_bin_times(bin_ms, heavy_ms);
             e->rec = nullptr;
             std::vector<unsigned long long> h(e->n);
             HIPCHK(hipMemcpyAsync(h.data(), rec.p, e->n * 8, hipMemcpyDeviceToHost, e->stream));
             e->sync();
+            // Lane-cycles measure a vertex's latency, not what it costs the
+            // device (waves resident per CU differ between kernels): every
+            // bin's (and heavy set's) measured time is shared among its
+            // vertices in proportion to their lane-cycles.  cost = ps.
+            auto vertex_at = [&](bool csr, uint32_t q) -> uint32_t {
+                return csr ? e->h_vertices[e->n_c + e->order_s[q]] : e->h_vertices[e->order_c[q]];
+            };
+            auto share = [&](float ms, const std::vector<const Bin *> &bs) {
+                long double sum = 0;
+                for (const Bin *b : bs)
+                    for (uint32_t q = b->begin; q < b->end; q++) sum += h[vertex_at(b->csr, q)];
+                if (sum <= 0) return;
+                const long double scale = (long double)ms * 1e9L / sum;
+                for (const Bin *b : bs)
+                    for (uint32_t q = b->begin; q < b->end; q++) {
+                        const uint32_t v = vertex_at(b->csr, q);
+                        h[v] = (unsigned long long)std::llround((long double)h[v] * scale);
+                    }
+            };
+            for (size_t i = 0; i < e->bins.size(); i++)
+                if (bin_ms[i] > 0) share(bin_ms[i], {&e->bins[i]});
+            for (size_t j = 0; j < e->heavy.size(); j++) {
+                if (heavy_ms[j] <= 0) continue;
+                const HeavySet &hs = *e->heavy[j];
+                std::vector<const Bin *> bs;
+                for (const Bin &b : e->bins)
+                    if (b.csr == hs.csr_part && b.kind == (hs.cms ? LPA_BIN_CMS : LPA_BIN_GLOBAL)) bs.push_back(&b);
+                share(heavy_ms[j], bs);
+            }
             for (uint64_t v = 0; v < e->n; v++) {
                 cost[v] = h[v];
                 measured[v] = family_of(byv[v]) == family;
@@ -1484,6 +1569,35 @@ int lpa_last_kernel_times(lpa_engine *e, float *ms_per_kind) {
     return guard([&] {
         if (!e || !ms_per_kind) throw Fail(LPA_ERR_ARG, "null argument");
         std::memcpy(ms_per_kind, e->kind_ms, sizeof e->kind_ms);
+    });
+}
+
+int lpa_calibration_copy(lpa_engine *e, uint64_t bytes, int reps, float *ms) {
+    return guard([&] {
+        if (!e || !ms) throw Fail(LPA_ERR_ARG, "null argument");
+        if (bytes == 0 || bytes % 16 || reps <= 0) throw Fail(LPA_ERR_RANGE, "bytes must be a positive multiple of 16");
+        HIPCHK(hipSetDevice(e->device));
+        const uint64_t n = bytes / 16;
+        DevBuf<uint4> src, dst;
+        src.alloc(n);
+        dst.alloc(n);
+        HIPCHK(hipMemsetAsync(src.p, 1, bytes, e->stream));
+        hipEvent_t a, b;
+        HIPCHK(hipEventCreate(&a));
+        HIPCHK(hipEventCreate(&b));
+        HIPCHK(hipEventRecord(a, e->stream));
+        for (int r = 0; r < reps; r++)
+            hipLaunchKernelGGL(k_calibration_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream, src.p,
+                               dst.p, n);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(b, e->stream));
+        HIPCHK(hipEventSynchronize(b));
+        float t = 0;
+        HIPCHK(hipEventElapsedTime(&t, a, b));
+        *ms = t / reps;
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        e->sync();
     });
 }
 

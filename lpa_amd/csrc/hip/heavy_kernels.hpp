@@ -76,6 +76,8 @@ __device__ __forceinline__ uint32_t heavy_deg(const HeavyArgs &h, uint32_t i) {
 // the staging buffer.
 constexpr uint32_t DECODE_STAGE = 8192;
 
+// CMS: the heavy set's mode (names the launch in profiles; same code)
+template <int CMS>
 __global__ void __launch_bounds__(BLOCK) k_heavy_decode(HeavyArgs h, const uint2 *__restrict__ tasks,
                                                         uint32_t ntasks, unsigned long long *__restrict__ cursor) {
     __shared__ GroupShared<BLOCK> sh;
@@ -150,6 +152,7 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_decode(HeavyArgs h, const uint2
 }
 
 // chunked[] lists the heavy-vertex indices handled by the chunk kernels.
+template <int CMS>
 __global__ void __launch_bounds__(BLOCK) k_heavy_clear(HeavyArgs h, const uint32_t *__restrict__ chunked,
                                                        uint32_t nchunked) {
     if (blockIdx.x >= nchunked) return;
@@ -397,6 +400,7 @@ __global__ void __launch_bounds__(HBLOCK) k_heavy_local(HeavyArgs h, const uint3
     if (a.rec && threadIdx.x == 0) atomicAdd(&a.rec[v], (cycles() - t0) * HBLOCK);
 }
 
+template <int CMS>
 __global__ void __launch_bounds__(BLOCK) k_heavy_finish(HeavyArgs h, const uint32_t *__restrict__ chunked,
                                                         uint32_t nchunked) {
     const uint64_t t0 = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;

@@ -395,16 +395,14 @@ __device__ __forceinline__ void count_staged(const StepArgs &a, const uint32_t *
     }
 }
 
-// Neighbours staged per lane by the WAVE / BLOCK kernels: up to 8 ids per
-// lane (quarter wave <= 128, wave <= 512, workgroup <= 2048 ids) and at most
-// 32 staged labels per lane, which keeps enough waves resident to hide the
-// decode and gather latency; the rest of a list is gathered per vector.
-template <int KB> constexpr int group_ipl() { return KB >= 8 ? 4 : 8; }
 
 // One group of G lanes per vertex; dynamic LDS holds, per group, 2^logT
 // table slots followed by the neighbour list (listcap ids).  The record is
-// decoded once; label vectors are counted KB at a time from staged rows.
-template <int G, bool CSR, int KB>
+// decoded once; label vectors are counted KB at a time from label rows
+// staged in registers, IPL neighbours per lane (the launch picks the
+// smallest IPL with IPL * G >= listcap, so the staged labels, IPL * KB
+// registers, cost no more occupancy than the bin needs).
+template <int G, bool CSR, int KB, int IPL>
 __global__ void __launch_bounds__(BLOCK) k_group(StepArgs a, uint32_t logT, uint32_t listcap) {
     constexpr int NG = BLOCK / G;
     extern __shared__ uint64_t lds[];
@@ -436,7 +434,7 @@ __global__ void __launch_bounds__(BLOCK) k_group(StepArgs a, uint32_t logT, uint
         for (uint32_t k0 = 0; k0 < a.K; k0 += KB) {
             const uint32_t kb = a.K - k0 < (uint32_t)KB ? a.K - k0 : (uint32_t)KB;
             uint64_t best = 0;
-            count_staged<G, group_ipl<KB>(), KB>(
+            count_staged<G, IPL, KB>(
                 a, ids, d, gl, k0, kb,
                 [&](int) {
                     for (uint32_t i = gl; i < T; i += G) tab[i] = EMPTY_SLOT;

@@ -175,6 +175,13 @@ __global__ void k_gather_u4(const uint4 *__restrict__ src, const uint32_t *__res
     if (i < n) out[i] = src[idx[i]];
 }
 
+// plain copy, 16 bytes per lane (counter calibration)
+__global__ void __launch_bounds__(256) k_calibration_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                          uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 __global__ void k_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;

@@ -52,6 +52,7 @@ EXPORTS = {
     "lpa_mlp_train": (C.c_int, [_VP, _VP, _VP, _VP, C.c_uint64, C.c_int, C.c_int, C.c_float, C.c_uint64, _VP]),
     "lpa_schedule_learned": (C.c_int, [_VP, _VP]),
     "lpa_last_kernel_times": (C.c_int, [_VP, _VP]),
+    "lpa_calibration_copy": (C.c_int, [_VP, C.c_uint64, C.c_int, _VP]),
 }
 
 
@@ -243,6 +244,12 @@ class Engine:
         t = np.zeros(6, np.float32)
         _check(self._L.lpa_last_kernel_times(self._h, _p(t)))
         return dict(zip(BIN_NAMES, t.tolist()))
+
+    def calibration_copy(self, nbytes=256 << 20, reps=3):
+        """plain device copy of nbytes, reps times; returns ms per copy"""
+        ms = C.c_float(0)
+        _check(self._L.lpa_calibration_copy(self._h, nbytes, reps, C.byref(ms)))
+        return ms.value
 
 
 MLP_NPARAM = 1443

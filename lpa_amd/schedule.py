@@ -1,10 +1,12 @@
 """Learned schedule: record per-vertex work, train the cost model, bin.
 
 The engine records, for one step with every vertex forced into one kernel
-family, the lane-cycles each vertex took (lpa_record_family).  The model
-(8 -> 32 -> 32 -> 3, trained on the device) learns log2(1 + lane-cycles)
-per family from the vertex's CGR features; the learned schedule puts every
-vertex in the family of least predicted cost.  Results do not depend on the
+family, each vertex's share of the device time (lpa_record_family: every
+bin's measured time split among its vertices in proportion to the
+lane-cycles they took, in picoseconds).  The model (8 -> 32 -> 32 -> 3,
+trained on the device) learns log2(1 + cost) per family from the vertex's
+CGR features; the learned schedule puts every vertex in the family of
+least predicted cost.  Results do not depend on the
 schedule: every family computes the same labels.
 """
 import numpy as np
