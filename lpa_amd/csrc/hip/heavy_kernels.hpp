@@ -271,6 +271,7 @@ __global__ void __launch_bounds__(HBLOCK) k_heavy_chunks(HeavyArgs h) {
                     best = 0;
                     __syncthreads();
                 },
+                [&](int, uint32_t lab, uint32_t cnt) { (void)tab_add(ltab, LOG_LT, lab, cnt); },
                 [&](int, uint32_t lab) {
                     const uint32_t cnt = fold_first_label<HBLOCK>(lab);
                     if (cnt) (void)tab_add(ltab, LOG_LT, lab, cnt);
@@ -382,6 +383,11 @@ __global__ void __launch_bounds__(HBLOCK) k_heavy_local(HeavyArgs h, const uint3
                     for (uint32_t x = threadIdx.x; x < T; x += HBLOCK) tab[x] = EMPTY_SLOT;
                     best = 0;
                     __syncthreads();
+                },
+                [&](int, uint32_t lab, uint32_t cnt) {
+                    const uint32_t c = tab_add(tab, logT, lab, cnt);
+                    const uint64_t key = ((uint64_t)c << 32) | (uint32_t)~lab;
+                    best = key > best ? key : best;
                 },
                 [&](int, uint32_t lab) {
                     const uint32_t cnt = fold_first_label<HBLOCK>(lab);

@@ -356,18 +356,60 @@ __device__ __forceinline__ uint32_t group_decode(const StepArgs &a, uint32_t p, 
     return sh->cnt;
 }
 
+// Aggregated counting rounds (per wave, or per sub-wave group): the group's
+// first lane still holding an uncounted label offers it, every copy of it
+// among the group's staged labels is found with one ballot per register and
+// the total is added by that lane with a single table update.  Rounds stop
+// when the wave has nothing left, after MATCH_ROUNDS, or when no group of
+// the wave matched MATCH_MIN copies (diverse neighbourhoods go one label at
+// a time instead).  Neighbourhoods that share a few labels -- most of them
+// once the labels settle -- are counted in a round or two.
+#ifndef LPA_MATCH_ROUNDS
+#define LPA_MATCH_ROUNDS 8
+#endif
+constexpr int MATCH_ROUNDS = LPA_MATCH_ROUNDS;
+constexpr uint32_t MATCH_MIN = 4;
+
+template <int G, int IPL, int KB, class Agg>
+__device__ __forceinline__ void match_rounds(const uint32_t (&labs)[IPL][KB], int s, uint32_t &pm, Agg &&agg) {
+    const uint64_t gm = group_mask<G>();
+    const uint32_t lane = __lane_id();
+    for (int r = 0; r < MATCH_ROUNDS; r++) {
+        const uint64_t wact = __ballot(pm != 0);
+        if (wact == 0) break;
+        const uint64_t act = wact & gm;
+        uint32_t mine = UNL;
+#pragma unroll
+        for (int t = IPL - 1; t >= 0; t--)
+            if (pm & (1u << t)) mine = labs[t][s];
+        const uint32_t leader = act ? (uint32_t)__builtin_ctzll(act) : lane;
+        const uint32_t lead = (uint32_t)__shfl((int)mine, (int)leader, 64);
+        uint32_t c = 0;
+#pragma unroll
+        for (int t = 0; t < IPL; t++) {
+            const bool m = (pm & (1u << t)) && labs[t][s] == lead;
+            c += (uint32_t)__popcll(__ballot(m) & gm);
+            if (m) pm &= ~(1u << t);
+        }
+        const bool lead_lane = act != 0 && lane == leader;
+        if (lead_lane) agg(s, lead, c);
+        if (__ballot(lead_lane && c >= MATCH_MIN) == 0) break;
+    }
+}
+
 // Phase B: counts label vectors k0 .. k0+kb-1 (kb <= KB, the same in every
 // lane of the group) of the neighbours ids[0..d), one vector at a time.
 // Lane gl first gathers the label-row slices of neighbours gl, gl+G, ...,
 // gl+(IPL-1)G into registers (IPL*KB values, all loads in flight together),
 // so a vertex's neighbour rows are read once per batch of KB vectors;
 // neighbours beyond IPL*G are gathered per vector.  For each vector s:
-// begin(s) prepares the table (and synchronises the group), use(s, lab)
-// is called for every labelled neighbour, end(s) finishes the vector.
-template <int G, int IPL, int KB, class Begin, class Use, class End>
+// begin(s) prepares the table (and synchronises the group); staged labels
+// go through aggregated rounds (agg(s, lab, count) by one lane per match)
+// and then use(s, lab) one at a time; end(s) finishes the vector.
+template <int G, int IPL, int KB, class Begin, class Agg, class Use, class End>
 __device__ __forceinline__ void count_staged(const StepArgs &a, const uint32_t *__restrict__ ids, uint32_t d,
-                                             uint32_t gl, uint32_t k0, uint32_t kb, Begin &&begin, Use &&use,
-                                             End &&end) {
+                                             uint32_t gl, uint32_t k0, uint32_t kb, Begin &&begin, Agg &&agg,
+                                             Use &&use, End &&end) {
     uint32_t labs[IPL][KB];
 #pragma unroll
     for (int t = 0; t < IPL; t++) {
@@ -383,9 +425,13 @@ __device__ __forceinline__ void count_staged(const StepArgs &a, const uint32_t *
     for (int s = 0; s < KB; s++) {
         if ((uint32_t)s < kb) {
             begin(s);
+            uint32_t pm = 0;
+#pragma unroll
+            for (int t = 0; t < IPL; t++) pm |= (labs[t][s] != UNL ? 1u : 0u) << t;
+            match_rounds<G, IPL, KB>(labs, s, pm, agg);
 #pragma unroll
             for (int t = 0; t < IPL; t++)
-                if (labs[t][s] != UNL) use(s, labs[t][s]);
+                if (pm & (1u << t)) use(s, labs[t][s]);
             for (uint32_t i = IPL * G + gl; i < d; i += G) {
                 const uint32_t lab = clean_label(a.lin[(uint64_t)ids[i] * a.K + k0 + s]);
                 if (lab != UNL) use(s, lab);
@@ -394,7 +440,6 @@ __device__ __forceinline__ void count_staged(const StepArgs &a, const uint32_t *
         }
     }
 }
-
 
 // One group of G lanes per vertex; dynamic LDS holds, per group, 2^logT
 // table slots followed by the neighbour list (listcap ids).  The record is
@@ -440,6 +485,11 @@ __global__ void __launch_bounds__(BLOCK) k_group(StepArgs a, uint32_t logT, uint
                     for (uint32_t i = gl; i < T; i += G) tab[i] = EMPTY_SLOT;
                     best = 0;
                     group_sync<G>();
+                },
+                [&](int, uint32_t lab, uint32_t cnt) {
+                    const uint32_t c = tab_add(tab, logT, lab, cnt);
+                    const uint64_t key = ((uint64_t)c << 32) | (uint32_t)~lab;
+                    best = key > best ? key : best;
                 },
                 [&](int, uint32_t lab) {
                     const uint32_t cnt = fold_first_label<G>(lab);
