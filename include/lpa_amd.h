@@ -73,14 +73,16 @@ enum {
     LPA_BIN_LANE = 1,       /* one lane per vertex, register sort (deg <= 32) */
     LPA_BIN_WAVE = 2,       /* one wave per vertex, LDS hash */
     LPA_BIN_BLOCK = 3,      /* one workgroup per vertex, LDS hash */
-    LPA_BIN_GLOBAL = 4,     /* one workgroup per vertex, hash table in HBM */
-    LPA_BIN_CMS = 5,        /* one workgroup per vertex, count-min sketch (forced by degree) */
+    LPA_BIN_GLOBAL = 4,     /* heavy exact: one 1024-lane workgroup per vertex (LDS table), or
+                               several per hub (chunks, HBM table) */
+    LPA_BIN_CMS = 5,        /* heavy count-min sketch (forced by degree): one workgroup per vertex
+                               (LDS sketches), or several per hub */
     LPA_NBIN_KINDS = 6
 };
 
 typedef struct {
     uint32_t lane_max;      /* deg <= lane_max -> LANE (<= 32) */
-    uint32_t wave_max;      /* deg <= wave_max -> WAVE */
+    uint32_t wave_max;      /* deg <= wave_max -> WAVE (0: 512) */
     uint32_t block_max;     /* deg <= block_max -> BLOCK, else the heavy-vertex path (0: LDS limit) */
     uint32_t relayout;      /* records are always reordered so each bin is contiguous; keep 1 */
     uint32_t quarter_max;   /* WAVE vertices up to this degree use 16-lane groups (0: 128) */

@@ -180,7 +180,7 @@ struct lpa_engine {
 
     // kind of every vertex under the current schedule (by vertex id)
     std::vector<uint8_t> kind_of;
-    lpa_sched_params sched_used{32, 256, 0, 1};
+    lpa_sched_params sched_used{32, 512, 0, 1, 128};
     // work recorder (lane-cycles per vertex id), null when off
     unsigned long long *rec = nullptr;
 
@@ -255,8 +255,10 @@ static int wave_groups(uint32_t logT, uint32_t quarter_max = 128) {
     return (1u << (logT - 1)) <= quarter_max ? 16 : 4;
 }
 
-// label vectors staged per row gather by the group kernels
-static int group_kb(uint32_t K) { return K >= 5 ? 8 : K >= 3 ? 4 : K == 2 ? 2 : 1; }
+// label vectors staged per row gather by the group kernels (4 keeps the
+// staged registers low enough for four waves per SIMD; the second half of a
+// 32-byte row is an L2 hit when K = 8)
+static int group_kb(uint32_t K) { return K >= 3 ? 4 : K == 2 ? 2 : 1; }
 
 // instance of the group kernel for a batch size and staged ids per lane
 template <int G, bool CSR> static const void *group_fn(int kb, uint32_t ipl) {
@@ -340,7 +342,7 @@ void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
     const bool one = K == 1;
     const size_t cells = (size_t)cms_depth * cms_width;
     h.kbs = (uint32_t)std::max<size_t>(
-        1, std::min<size_t>({(size_t)K, (size_t)(one ? 1 : HEAVY_KB_MAX), (size_t)(lds_max - 20480) / (cells * 4)}));
+        1, std::min<size_t>({(size_t)K, (size_t)(one ? 1 : HEAVY_KB_CMS), (size_t)(lds_max - 20480) / (cells * 4)}));
     if (!as_csr) {
         hipLaunchKernelGGL(k_fill_u64, dim3((hs.nh + 255) / 256), dim3(256), 0, stream, hs.cursor.p, (uint64_t)hs.nh, 0ull);
         if (hs.cms)
@@ -355,7 +357,7 @@ void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
         const size_t sk = h.kbs * cells * sizeof(uint32_t), tb = 2 * CHUNK * sizeof(uint64_t);
         if (hs.cms) {
             if (one) hipLaunchKernelGGL((k_heavy_local<1, 1>), g, b, sk, stream, h, hs.local.p, hs.nlocal);
-            else hipLaunchKernelGGL((k_heavy_local<1, HEAVY_KB_MAX>), g, b, sk, stream, h, hs.local.p, hs.nlocal);
+            else hipLaunchKernelGGL((k_heavy_local<1, HEAVY_KB_CMS>), g, b, sk, stream, h, hs.local.p, hs.nlocal);
         } else {
             if (one) hipLaunchKernelGGL((k_heavy_local<0, 1>), g, b, tb, stream, h, hs.local.p, hs.nlocal);
             else hipLaunchKernelGGL((k_heavy_local<0, HEAVY_KB_MAX>), g, b, tb, stream, h, hs.local.p, hs.nlocal);
@@ -374,8 +376,8 @@ void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
                 hipLaunchKernelGGL((k_heavy_chunks<1, 1>), g, b, sk, stream, h);
                 hipLaunchKernelGGL((k_heavy_chunks<2, 1>), g, b, 0, stream, h);
             } else {
-                hipLaunchKernelGGL((k_heavy_chunks<1, HEAVY_KB_MAX>), g, b, sk, stream, h);
-                hipLaunchKernelGGL((k_heavy_chunks<2, HEAVY_KB_MAX>), g, b, 0, stream, h);
+                hipLaunchKernelGGL((k_heavy_chunks<1, HEAVY_KB_CMS>), g, b, sk, stream, h);
+                hipLaunchKernelGGL((k_heavy_chunks<2, HEAVY_KB_CMS>), g, b, 0, stream, h);
             }
         } else {
             const size_t tb = 2 * CHUNK * sizeof(uint64_t);
@@ -504,9 +506,9 @@ int lpa_engine_create(int device, lpa_engine **out) {
             allow_group_lds<64>(dyn);
             allow_group_lds<BLOCK>(dyn);
             const void *heavy_fns[] = {(const void *)k_heavy_chunks<0, 1>, (const void *)k_heavy_chunks<0, HEAVY_KB_MAX>,
-                                       (const void *)k_heavy_chunks<1, 1>, (const void *)k_heavy_chunks<1, HEAVY_KB_MAX>,
+                                       (const void *)k_heavy_chunks<1, 1>, (const void *)k_heavy_chunks<1, HEAVY_KB_CMS>,
                                        (const void *)k_heavy_local<0, 1>, (const void *)k_heavy_local<0, HEAVY_KB_MAX>,
-                                       (const void *)k_heavy_local<1, 1>, (const void *)k_heavy_local<1, HEAVY_KB_MAX>};
+                                       (const void *)k_heavy_local<1, 1>, (const void *)k_heavy_local<1, HEAVY_KB_CMS>};
             for (const void *f : heavy_fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
             (void)hipGetLastError();
         } catch (...) {
@@ -877,7 +879,7 @@ int lpa_schedule_degree(lpa_engine *e, const lpa_sched_params *p) {
         if (p->lane_max > 32) throw Fail(LPA_ERR_RANGE, "lane_max must be <= 32");
         HIPCHK(hipSetDevice(e->device));
         lpa_sched_params q = *p;
-        if (q.wave_max == 0) q.wave_max = 256;
+        if (q.wave_max == 0) q.wave_max = 512;
         if (q.block_max == 0) q.block_max = block_cap(e);
         auto kind = [&](uint32_t d) -> uint8_t {
             int want = d <= q.lane_max ? LPA_BIN_LANE : d <= q.wave_max ? LPA_BIN_WAVE

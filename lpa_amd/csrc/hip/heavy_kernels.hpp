@@ -185,8 +185,10 @@ constexpr int HEAVY_IPL = CHUNK / HBLOCK;
 // serve up to KB label vectors (kbs sketches side by side) per round.
 constexpr int CMS_R = 4;
 // Label vectors per staged batch of the heavy kernels (template KB): 1 for
-// a single vector, else 8.
+// a single vector, else 8 for exact counting and 4 for sketches (four
+// sketches side by side leave LDS for two workgroups per CU).
 constexpr int HEAVY_KB_MAX = 8;
+constexpr int HEAVY_KB_CMS = 4;
 
 // One sketch pass over ids[0..n) for vectors k0 .. k0+kb-1 (kb <= KB;
 // sketch s at sk + s*cells).  ADD: add every label to its sketch (LDS,

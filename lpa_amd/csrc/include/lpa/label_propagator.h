@@ -32,7 +32,7 @@ struct LpOptions {
     uint32_t cms_depth = 4, cms_width = 1024;
     uint64_t cms_seed = 0;
     bool until_converged = false;   // stop early when no label changes
-    lpa_sched_params sched{32, 256, 0, 1, 128};
+    lpa_sched_params sched{32, 512, 0, 1, 128};
     std::vector<uint8_t> bins;      // explicit per-vertex bins (learned schedule); empty: degree
     std::vector<uint32_t> init;     // initial labels (n*K); empty: L0(v) = v
     bool verbose = true;

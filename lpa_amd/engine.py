@@ -152,7 +152,7 @@ class Engine:
         _check(self._L.lpa_lp_config(self._h, K, int(semi), cms_thresh, cms_depth, cms_width, cms_seed))
         self.K = K
 
-    def schedule_degree(self, lane_max=32, wave_max=256, block_max=0, quarter_max=128):
+    def schedule_degree(self, lane_max=32, wave_max=512, block_max=0, quarter_max=128):
         p = SchedParams(lane_max, wave_max, block_max, 1, quarter_max)
         _check(self._L.lpa_schedule_degree(self._h, C.byref(p)))
 
