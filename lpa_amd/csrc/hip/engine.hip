@@ -92,8 +92,10 @@ struct HeavySet {
     bool csr_part = false;
     bool cms = false;
     uint32_t nh = 0, nchunks = 0, kb = 1;
-    uint32_t nlocal = 0, nchunked = 0;
-    DevBuf<uint32_t> local, chunked, slot;   // heavy-vertex indices: one workgroup each / chunked; slots
+    uint32_t nlocal = 0, nlocal_big = 0, nchunked = 0;
+    // heavy-vertex indices: one workgroup each (the last nlocal_big of them
+    // exact with degree in (CHUNK, 2*CHUNK]) / chunked; chunk slots
+    DevBuf<uint32_t> local, chunked, slot;
     DevBuf<uint32_t> pos, chunk_h;
     DevBuf<uint64_t> loff, chunk_b, toff;
     DevBuf<uint32_t> list;
@@ -353,14 +355,26 @@ void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
                                hs.cursor.p);
     }
     if (hs.nlocal) {
-        const dim3 g(hs.nlocal), b(HBLOCK);
-        const size_t sk = h.kbs * cells * sizeof(uint32_t), tb = 2 * CHUNK * sizeof(uint64_t);
+        const dim3 b(HBLOCK);
+        const size_t sk = h.kbs * cells * sizeof(uint32_t);
+        const uint32_t nsmall = hs.nlocal - hs.nlocal_big;
+        const uint32_t *lsmall = hs.local.p, *lbig = hs.local.p + nsmall;
         if (hs.cms) {
-            if (one) hipLaunchKernelGGL((k_heavy_local<1, 1>), g, b, sk, stream, h, hs.local.p, hs.nlocal);
-            else hipLaunchKernelGGL((k_heavy_local<1, HEAVY_KB_CMS>), g, b, sk, stream, h, hs.local.p, hs.nlocal);
+            const dim3 g(hs.nlocal);
+            if (one) hipLaunchKernelGGL((k_heavy_local<1, 1, 4>), g, b, sk, stream, h, hs.local.p, hs.nlocal);
+            else hipLaunchKernelGGL((k_heavy_local<1, HEAVY_KB_CMS, 4>), g, b, sk, stream, h, hs.local.p, hs.nlocal);
         } else {
-            if (one) hipLaunchKernelGGL((k_heavy_local<0, 1>), g, b, tb, stream, h, hs.local.p, hs.nlocal);
-            else hipLaunchKernelGGL((k_heavy_local<0, HEAVY_KB_MAX>), g, b, tb, stream, h, hs.local.p, hs.nlocal);
+            // tables of up to 2*CHUNK (8192 ids: 4 per lane) or 4*CHUNK slots (8 per lane)
+            const size_t t1 = 2 * CHUNK * sizeof(uint64_t), t2 = 4 * CHUNK * sizeof(uint64_t);
+            if (nsmall) {
+                if (one) hipLaunchKernelGGL((k_heavy_local<0, 1, 4>), dim3(nsmall), b, t1, stream, h, lsmall, nsmall);
+                else hipLaunchKernelGGL((k_heavy_local<0, HEAVY_KB_MAX, 4>), dim3(nsmall), b, t1, stream, h, lsmall, nsmall);
+            }
+            if (hs.nlocal_big) {
+                const dim3 g(hs.nlocal_big);
+                if (one) hipLaunchKernelGGL((k_heavy_local<0, 1, 8>), g, b, t2, stream, h, lbig, hs.nlocal_big);
+                else hipLaunchKernelGGL((k_heavy_local<0, HEAVY_KB_CMS, 8>), g, b, t2, stream, h, lbig, hs.nlocal_big);
+            }
         }
     }
     if (hs.nchunked == 0) return;
@@ -507,8 +521,9 @@ int lpa_engine_create(int device, lpa_engine **out) {
             allow_group_lds<BLOCK>(dyn);
             const void *heavy_fns[] = {(const void *)k_heavy_chunks<0, 1>, (const void *)k_heavy_chunks<0, HEAVY_KB_MAX>,
                                        (const void *)k_heavy_chunks<1, 1>, (const void *)k_heavy_chunks<1, HEAVY_KB_CMS>,
-                                       (const void *)k_heavy_local<0, 1>, (const void *)k_heavy_local<0, HEAVY_KB_MAX>,
-                                       (const void *)k_heavy_local<1, 1>, (const void *)k_heavy_local<1, HEAVY_KB_CMS>};
+                                       (const void *)k_heavy_local<0, 1, 4>, (const void *)k_heavy_local<0, HEAVY_KB_MAX, 4>,
+                                       (const void *)k_heavy_local<0, 1, 8>, (const void *)k_heavy_local<0, HEAVY_KB_CMS, 8>,
+                                       (const void *)k_heavy_local<1, 1, 4>, (const void *)k_heavy_local<1, HEAVY_KB_CMS, 4>};
             for (const void *f : heavy_fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
             (void)hipGetLastError();
         } catch (...) {
@@ -682,12 +697,14 @@ static void build_heavy(lpa_engine *e) {
             auto hs = std::make_unique<HeavySet>();
             hs->csr_part = part == 1;
             hs->cms = cms == 1;
-            std::vector<uint32_t> pos, local, chunked, slot;
+            std::vector<uint32_t> pos, local, local_big, chunked, slot;
             std::vector<uint64_t> loff(1, 0), toff(1, 0), cb;
             std::vector<uint32_t> ch;
-            // one workgroup holds the whole count: exact up to CHUNK ids (LDS
-            // table of 2*CHUNK slots), CMS up to CMS_LOCAL ids (LDS sketch)
-            const uint64_t local_max = cms ? CMS_LOCAL : CHUNK;
+            // one workgroup holds the whole count: exact up to 2*CHUNK ids
+            // (LDS table of up to 4*CHUNK slots, when the LDS holds it), CMS
+            // up to CMS_LOCAL ids (LDS sketches)
+            const bool big_ok = 4ull * CHUNK * 8 <= lds_budget(e);
+            const uint64_t local_max = cms ? CMS_LOCAL : big_ok ? 2 * CHUNK : CHUNK;
             for (const Bin &b : e->bins) {
                 if (b.csr != hs->csr_part) continue;
                 if (b.kind != (cms ? LPA_BIN_CMS : LPA_BIN_GLOBAL)) continue;
@@ -698,7 +715,9 @@ static void build_heavy(lpa_engine *e) {
                     pos.push_back(q);
                     const bool is_local = d <= local_max;
                     slot.push_back(is_local ? 0u : (uint32_t)chunked.size());
-                    (is_local ? local : chunked).push_back(i);
+                    if (!is_local) chunked.push_back(i);
+                    else if (!cms && d > CHUNK) local_big.push_back(i);
+                    else local.push_back(i);
                     if (!is_local)
                         for (uint64_t c0 = 0; c0 < d; c0 += CHUNK) {
                             ch.push_back(i);
@@ -710,6 +729,8 @@ static void build_heavy(lpa_engine *e) {
                 }
             }
             if (pos.empty()) continue;
+            hs->nlocal_big = (uint32_t)local_big.size();
+            local.insert(local.end(), local_big.begin(), local_big.end());
             hs->nh = (uint32_t)pos.size();
             hs->nchunks = (uint32_t)ch.size();
             hs->nlocal = (uint32_t)local.size();

@@ -328,13 +328,15 @@ __global__ void __launch_bounds__(HBLOCK) k_heavy_chunks(HeavyArgs h) {
 }
 
 // Heavy vertices small enough for one workgroup's LDS (exact: degree <=
-// CHUNK, table of 2^ceil(log2 2d) slots; CMS: any degree up to the local
-// limit, kbs sketches in LDS): one workgroup per vertex counts its whole
-// list -- both CMS passes, or the exact table -- and writes the labels
-// itself, with no HBM table, sketch or merge.  local[] lists the
-// heavy-vertex indices.
-template <int CMS, int KB>
-__global__ void __launch_bounds__(HBLOCK) k_heavy_local(HeavyArgs h, const uint32_t *__restrict__ local,
+// 2*CHUNK, table of 2^ceil(log2 2d) slots -- up to 128 KB; CMS: any degree
+// up to the local limit, kbs sketches in LDS): one workgroup per vertex
+// counts its whole list -- both CMS passes, or the exact table -- and writes
+// the labels itself, with no HBM table, sketch or merge.  local[] lists the
+// heavy-vertex indices; IPL = staged ids per lane (exact).
+// (Eight waves per SIMD requested: two 1024-lane workgroups per CU when their
+// tables fit, at the price of a few spilled registers in the widest case.)
+template <int CMS, int KB, int IPL>
+__global__ void __launch_bounds__(HBLOCK, 8) k_heavy_local(HeavyArgs h, const uint32_t *__restrict__ local,
                                                        uint32_t nlocal) {
     extern __shared__ uint64_t lds[];
     __shared__ uint64_t red[HBLOCK / 64];
@@ -379,7 +381,7 @@ __global__ void __launch_bounds__(HBLOCK) k_heavy_local(HeavyArgs h, const uint3
         for (uint32_t k0 = 0; k0 < a.K; k0 += KB) {
             const uint32_t kb = a.K - k0 < (uint32_t)KB ? a.K - k0 : (uint32_t)KB;
             uint64_t best = 0;
-            count_staged<HBLOCK, HEAVY_IPL, KB>(
+            count_staged<HBLOCK, IPL, KB>(
                 a, ids, d, threadIdx.x, k0, kb,
                 [&](int) {
                     for (uint32_t x = threadIdx.x; x < T; x += HBLOCK) tab[x] = EMPTY_SLOT;

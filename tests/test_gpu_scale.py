@@ -1,0 +1,90 @@
+"""Parity at the benchmark workloads' full sizes, through the C ABI.
+
+R-MAT scale 22 (the default benchmark graph, 4.2 M vertices, 128 M arcs)
+runs by default: ten steps with K = 1 and two with K = 8 (count-min sketch
+4 x 1024 above degree 4096), bit for bit against the oracle after every
+step.  R-MAT scale 27 (K = 8, 8.6 G arcs) and scale 24 (K = 64,
+semi-supervised) take minutes to generate on the host; with
+LPA_FULL_SCALE=1 their one-step checks run: the oracle recomputes L_{t+1}
+from the engine's L_t for a random sample of a million vertices, after the
+first and after the last step.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import lpa_amd as L
+import pyoracle as O
+from gpu_common import make_engine, oracle_view, run_and_compare
+
+pytestmark = pytest.mark.gpu
+
+FULL = os.environ.get("LPA_FULL_SCALE") == "1"
+CMS22 = dict(cms_thresh=4096, cms_depth=4, cms_width=1024, cms_seed=0)
+
+
+@pytest.fixture(scope="module")
+def rmat22():
+    g = L.gen_rmat(22, 16, 0.57, 0.19, 0.19, seed=1, bfs=True)
+    return g.encode(0, 128)
+
+
+def test_rmat22_ten_steps_k1(rmat22):
+    run_and_compare(rmat22, 10, **CMS22)
+
+
+def test_rmat22_two_steps_k8(rmat22):
+    run_and_compare(rmat22, 2, K=8, **CMS22)
+
+
+def one_step_checks(cgr, iters, check_at, K, semi=False, labels0=None, sample=1_000_000, **cms):
+    """runs `iters` engine steps; after each step in check_at, recomputes a
+    sample of vertices with the oracle from the engine's previous labels"""
+    e = make_engine(cgr, K=K, semi=semi, **cms)
+    e.labels_init(labels0)
+    view = oracle_view(cgr)
+    c = O.cfg(K=K, semi=semi, **cms)
+    seeds = None
+    if semi:
+        lab0 = np.ascontiguousarray(labels0, np.uint32)
+        seeds = (lab0 != O.UNLABELLED).astype(np.uint8)
+    rng = np.random.default_rng(7)
+    npos = cgr.n_c + len(cgr.small_offsets) - 1 if len(cgr.small_offsets) else cgr.n_c
+    for it in range(iters):
+        before = e.labels() if it in check_at else None
+        e.nova_iterate()
+        if before is None:
+            continue
+        after = e.labels()
+        pos = np.sort(rng.choice(npos, min(sample, npos), replace=False)).astype(np.uint64)
+        out = before.copy()
+        O.step_cgr_subset(view, before, pos, out, c, seeds)
+        verts = cgr.vertices[pos.astype(np.int64)].astype(np.int64)
+        idx = (verts[:, None] * K + np.arange(K)[None, :]).reshape(-1)
+        bad = np.nonzero(after[idx] != out[idx])[0]
+        print(f"[scale] step {it + 1}: {len(idx)} sampled labels, {len(bad)} differ", flush=True)
+        assert len(bad) == 0, (it, idx[bad[:5]], after[idx[bad[:5]]], out[idx[bad[:5]]])
+    e.close()
+
+
+@pytest.mark.skipif(not FULL, reason="R-MAT scale 27 takes minutes to build; set LPA_FULL_SCALE=1")
+def test_rmat27_k8_one_step_checks():
+    g = L.gen_rmat(27, 32, 0.57, 0.19, 0.19, seed=4, bfs=True)
+    cgr = g.encode(0, 128)
+    del g
+    one_step_checks(cgr, 10, (0, 9), 8, cms_thresh=8192, cms_depth=4, cms_width=2048, cms_seed=0)
+
+
+@pytest.mark.skipif(not FULL, reason="R-MAT scale 24 with K = 64 takes minutes; set LPA_FULL_SCALE=1")
+def test_rmat24_k64_semi_one_step_checks():
+    g = L.gen_rmat(24, 16, 0.57, 0.19, 0.19, seed=3, bfs=True)
+    cgr = g.encode(0, 128)
+    n, K = g.n, 64
+    del g
+    rng = np.random.default_rng(3)
+    lab = np.full((n, K), 0xFFFFFFFF, np.uint32)
+    for k in range(K):
+        idx = rng.choice(n, n // 100, replace=False)
+        lab[idx, k] = rng.integers(0, 100, len(idx), dtype=np.uint32)
+    one_step_checks(cgr, 15, (0, 14), K, semi=True, labels0=lab.reshape(-1), sample=200_000)
