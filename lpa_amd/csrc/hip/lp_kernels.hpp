@@ -249,20 +249,18 @@ __device__ __forceinline__ uint64_t max_u64_over(uint64_t x) {
 }
 
 // Add `cnt` occurrences of lab to the table (2^logT slots); returns the
-// label's count after the addition.
+// label's count after the addition.  A slot is one 64-bit word (label << 32
+// | count), cleared to EMPTY_SLOT; the label half is claimed with a 32-bit
+// compare-and-swap (which also finds a slot already holding the label) and
+// the count half takes a 32-bit atomic add, so every probe is one CAS and
+// the hit one more add, with no separate read.
 __device__ __forceinline__ uint32_t tab_add(uint64_t *tab, uint32_t logT, uint32_t lab, uint32_t cnt) {
     const uint32_t mask = (1u << logT) - 1;
     uint32_t h = logT ? (lab * 2654435769u) >> (32 - logT) : 0;
+    uint32_t *w = reinterpret_cast<uint32_t *>(tab);   // little endian: w[2h] count, w[2h+1] label
     for (;;) {
-        uint64_t e = __hip_atomic_load(&tab[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        uint32_t key = (uint32_t)(e >> 32);
-        if (key == lab) return (uint32_t)atomicAdd((unsigned long long *)&tab[h], (unsigned long long)cnt) + cnt;
-        if (key == UNL) {
-            uint64_t old = atomicCAS((unsigned long long *)&tab[h], EMPTY_SLOT, ((uint64_t)lab << 32) | cnt);
-            if (old == EMPTY_SLOT) return cnt;
-            if ((uint32_t)(old >> 32) == lab)
-                return (uint32_t)atomicAdd((unsigned long long *)&tab[h], (unsigned long long)cnt) + cnt;
-        }
+        const uint32_t old = atomicCAS(&w[2 * h + 1], UNL, lab);
+        if (old == UNL || old == lab) return atomicAdd(&w[2 * h], cnt) + cnt;
         h = (h + 1) & mask;
     }
 }
