@@ -198,6 +198,19 @@ template <bool ADD, int KB>
 __device__ __forceinline__ void cms_rounds(const StepArgs &a, const uint32_t *__restrict__ ids, uint32_t n,
                                            uint32_t k0, uint32_t kb, uint32_t *sk, uint32_t cells,
                                            uint64_t (&best)[KB]) {
+    // the row hashes stay in scalar registers for the whole pass (a runtime
+    // row index into the argument block would reload them per label)
+    constexpr int MAXD = 8;
+    uint64_t ha[MAXD], hb[MAXD];
+#pragma unroll
+    for (int r = 0; r < MAXD; r++) {
+        ha[r] = a.cms_ab[2 * r];
+        hb[r] = a.cms_ab[2 * r + 1];
+    }
+    const uint32_t depth = a.cms_depth, logw = a.cms_logw, sh = 64 - logw;
+    auto col = [&](int r, uint32_t lab) -> uint32_t {
+        return logw ? (uint32_t)((ha[r] * (uint64_t)lab + hb[r]) >> sh) : 0u;
+    };
     for (uint32_t i0 = 0; i0 < n; i0 += CMS_R * HBLOCK) {
         uint32_t labs[CMS_R][KB];
 #pragma unroll
@@ -221,12 +234,17 @@ __device__ __forceinline__ void cms_rounds(const StepArgs &a, const uint32_t *__
                 if (ADD) {
                     const uint32_t cnt = fold_first_label<HBLOCK>(lab);
                     if (cnt == 0) continue;
-                    for (uint32_t r = 0; r < a.cms_depth; r++)
-                        atomicAdd(&skq[(r << a.cms_logw) + cms_col(a, r, lab)], cnt);
+#pragma unroll
+                    for (int r = 0; r < MAXD; r++) {
+                        if ((uint32_t)r >= depth) break;
+                        atomicAdd(&skq[((uint32_t)r << logw) + col(r, lab)], cnt);
+                    }
                 } else {
                     uint32_t est = UNL;
-                    for (uint32_t r = 0; r < a.cms_depth; r++) {
-                        const uint32_t x = skq[(r << a.cms_logw) + cms_col(a, r, lab)];
+#pragma unroll
+                    for (int r = 0; r < MAXD; r++) {
+                        if ((uint32_t)r >= depth) break;
+                        const uint32_t x = skq[((uint32_t)r << logw) + col(r, lab)];
                         est = x < est ? x : est;
                     }
                     const uint64_t key = ((uint64_t)est << 32) | (uint32_t)~lab;
