@@ -107,6 +107,22 @@ int lpa_graph_upload(lpa_engine *e, uint64_t n, uint64_t n_c,
                      const uint32_t *words, uint64_t nwords, const uint64_t *offsets,
                      const uint32_t *vertices, const uint64_t *small_offsets,
                      const uint32_t *small_nbrs);
+/* Encodes a CSR (host arrays: row[n+1], col[row[n]], every row strictly
+ * increasing) into the CGR stream on the device and makes it the engine's
+ * graph, as lpa_graph_upload would with the same file set: vertex order
+ * deg >= 128 by id, then small_thresh <= deg < 128 by id, deg <
+ * small_thresh as the small CSR.  Replaces CgrCompressor::encode_node /
+ * write_cgr (src/compressor.cpp:157-515) run on the host; the bitstream is
+ * the reference compressor's, bit for bit.  Rejects unsorted rows,
+ * duplicate or out-of-range ids (LPA_ERR_FORMAT). */
+int lpa_graph_encode_csr(lpa_engine *e, uint64_t n, const uint64_t *row, const uint32_t *col,
+                         uint32_t small_thresh);
+/* The engine's compressed part as stored: *n_c, *nbits, and when not NULL
+ * words[ceil(nbits/64)] (64-bit words, bit 0 = MSB of word 0), offsets[n_c+1]
+ * (start bit of each record) and vertices[n] (stream order, then small) --
+ * what lpah_files_write needs to write the reference's file set. */
+int lpa_graph_stream(lpa_engine *e, uint64_t *n_c, uint64_t *nbits, uint64_t *words, uint64_t *offsets,
+                     uint32_t *vertices);
 /* deg[v] for every vertex id v (n entries), computed by decoding. */
 int lpa_graph_degrees(lpa_engine *e, uint32_t *deg);
 /* Per-vertex work features for the cost model, by vertex id, 8 floats each:

@@ -22,6 +22,7 @@
 #include <string>
 #include <vector>
 
+#include "encode_kernels.hpp"
 #include "heavy_kernels.hpp"
 #include "mlp_kernels.hpp"
 #include "lp_kernels.hpp"
@@ -484,6 +485,32 @@ void lpa_engine::step_bin_times(std::vector<float> &bin_ms, std::vector<float> &
     for (auto &x : ev) (void)hipEventDestroy(x);
 }
 
+// Validation pass over the device stream (words0, offs0, vid0 and the host
+// copies set): degrees, features, segment metadata; rejects malformed
+// records.
+static void finish_graph(lpa_engine *e) {
+    const uint64_t n_c = e->n_c;
+    DevBuf<uint32_t> ddeg;
+    ddeg.alloc(n_c);
+    e->feat0.alloc(n_c * NFEAT);
+    e->meta0.alloc(n_c);
+    e->counter.alloc(1);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, e->stream, e->counter.p, (uint64_t)1, 0ull);
+    if (n_c)
+        hipLaunchKernelGGL(k_scan_records, dim3((unsigned)((n_c + 255) / 256)), dim3(256), 0, e->stream, e->words0.p,
+                           e->offs0.p, e->vid0.p, n_c, (uint32_t)e->n, ddeg.p, e->feat0.p, e->meta0.p, e->counter.p);
+    HIPCHK(hipGetLastError());
+    unsigned long long bad = 0;
+    HIPCHK(hipMemcpyAsync(&bad, e->counter.p, sizeof bad, hipMemcpyDeviceToHost, e->stream));
+    e->deg_c.assign(n_c, 0);
+    if (n_c) HIPCHK(hipMemcpyAsync(e->deg_c.data(), ddeg.p, n_c * 4, hipMemcpyDeviceToHost, e->stream));
+    e->sync();
+    if (bad) throw Fail(LPA_ERR_FORMAT, "malformed CGR record at stream position " + std::to_string(bad - 1));
+    e->m_c = 0;
+    for (uint32_t d : e->deg_c) e->m_c += d;
+    e->have_graph = true;
+}
+
 // ------------------------------------------------------------------ C ABI
 
 static uint64_t lds_budget(const lpa_engine *e);
@@ -605,26 +632,102 @@ int lpa_graph_upload(lpa_engine *e, uint64_t n, uint64_t n_c, const uint32_t *wo
                            std::min<uint64_t>(nwords, (nbits + 31) / 32), e->words0.p, n64);
         e->offs0.upload(offsets, n_c + 1, e->stream);
         e->vid0.upload(vertices, n_c, e->stream);
-        DevBuf<uint32_t> ddeg;
-        ddeg.alloc(n_c);
-        e->feat0.alloc(n_c * NFEAT);
-        e->meta0.alloc(n_c);
+        finish_graph(e);
+    });
+}
+
+int lpa_graph_encode_csr(lpa_engine *e, uint64_t n, const uint64_t *row, const uint32_t *col,
+                         uint32_t small_thresh) {
+    return guard([&] {
+        if (!e || !row || (row[n] && !col)) throw Fail(LPA_ERR_ARG, "null argument");
+        if (n >= (1ull << 30)) throw Fail(LPA_ERR_RANGE, "n must be < 2^30");
+        if (row[0] != 0) throw Fail(LPA_ERR_FORMAT, "row[0] != 0");
+        for (uint64_t v = 0; v < n; v++)
+            if (row[v + 1] < row[v]) throw Fail(LPA_ERR_FORMAT, "row offsets decrease");
+        HIPCHK(hipSetDevice(e->device));
+        e->drop_graphs();
+        e->have_graph = e->scheduled = e->have_labels = e->have_csr = false;
+        const uint64_t m = row[n];
+        // stream order (compressor.cpp:157-290): deg >= 128 by id, then
+        // small_thresh <= deg < 128 by id; deg < small_thresh -> small CSR
+        const uint32_t large = 128;
+        std::vector<uint32_t> vert;
+        vert.reserve(n);
+        auto deg = [&](uint64_t v) { return row[v + 1] - row[v]; };
+        for (uint64_t v = 0; v < n; v++) if (deg(v) >= large) vert.push_back((uint32_t)v);
+        for (uint64_t v = 0; v < n; v++) if (deg(v) >= small_thresh && deg(v) < large) vert.push_back((uint32_t)v);
+        const uint64_t n_c = vert.size();
+        for (uint64_t v = 0; v < n; v++) if (deg(v) < small_thresh) vert.push_back((uint32_t)v);
+        const uint64_t n_s = n - n_c;
+        e->h_soffs.assign(1, 0);
+        e->h_snbr.clear();
+        for (uint64_t j = n_c; j < n; j++) {
+            const uint32_t v = vert[j];
+            for (uint64_t k = row[v]; k < row[v + 1]; k++) {
+                if (col[k] >= n) throw Fail(LPA_ERR_FORMAT, "neighbour id >= n");
+                e->h_snbr.push_back(col[k]);
+            }
+            e->h_soffs.push_back(e->h_snbr.size());
+        }
+        e->n = n;
+        e->n_c = n_c;
+        e->n_s = n_s;
+        e->m_s = e->h_snbr.size();
+        e->h_vertices = vert;
+        // encode on the device
+        DevBuf<uint64_t> drow, len, offs;
+        DevBuf<uint32_t> dcol;
+        drow.upload(row, n + 1, e->stream);
+        dcol.upload(col, m, e->stream);
+        e->vid0.upload(vert.data(), n_c, e->stream);
+        len.alloc(n_c);
+        e->offs0.alloc(n_c + 1);
         e->counter.alloc(1);
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, e->stream, e->counter.p, (uint64_t)1, 0ull);
         if (n_c)
-            hipLaunchKernelGGL(k_scan_records, dim3((unsigned)((n_c + 255) / 256)), dim3(256), 0, e->stream,
-                               e->words0.p, e->offs0.p, e->vid0.p, n_c, (uint32_t)n, ddeg.p, e->feat0.p,
-                               e->meta0.p, e->counter.p);
+            hipLaunchKernelGGL(k_encode_len, dim3((unsigned)((n_c + 255) / 256)), dim3(256), 0, e->stream, drow.p,
+                               dcol.p, e->vid0.p, n_c, (uint32_t)n, len.p, e->counter.p);
+        const uint64_t nb = std::max<uint64_t>(1, (n_c + SCAN_PER_BLOCK - 1) / SCAN_PER_BLOCK);
+        DevBuf<uint64_t> sums;
+        sums.alloc(nb);
+        hipLaunchKernelGGL(k_scan_sums, dim3((unsigned)nb), dim3(256), 0, e->stream, len.p, n_c, sums.p);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, e->stream, sums.p, nb);
+        hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(256), 0, e->stream, len.p, n_c, sums.p,
+                           e->offs0.p, nb);
         HIPCHK(hipGetLastError());
         unsigned long long bad = 0;
         HIPCHK(hipMemcpyAsync(&bad, e->counter.p, sizeof bad, hipMemcpyDeviceToHost, e->stream));
-        e->deg_c.assign(n_c, 0);
-        if (n_c) HIPCHK(hipMemcpyAsync(e->deg_c.data(), ddeg.p, n_c * 4, hipMemcpyDeviceToHost, e->stream));
+        e->h_offs.assign(n_c + 1, 0);
+        HIPCHK(hipMemcpyAsync(e->h_offs.data(), e->offs0.p, (n_c + 1) * 8, hipMemcpyDeviceToHost, e->stream));
         e->sync();
-        if (bad) throw Fail(LPA_ERR_FORMAT, "malformed CGR record at stream position " + std::to_string(bad - 1));
-        e->m_c = 0;
-        for (uint32_t d : e->deg_c) e->m_c += d;
-        e->have_graph = true;
+        if (bad) throw Fail(LPA_ERR_FORMAT, "row of vertex " + std::to_string(bad - 1) +
+                                                " is not strictly increasing or has an id >= n");
+        e->nbits = e->h_offs[n_c];
+        const uint64_t n64 = (e->nbits + 63) / 64 + 8;
+        e->words0.alloc(n64);
+        hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((n64 + 255) / 256)), dim3(256), 0, e->stream,
+                           (unsigned long long *)e->words0.p, n64, 0ull);
+        if (n_c)
+            hipLaunchKernelGGL(k_encode_write, dim3((unsigned)((n_c + 255) / 256)), dim3(256), 0, e->stream, drow.p,
+                               dcol.p, e->vid0.p, n_c, e->offs0.p, e->words0.p);
+        HIPCHK(hipGetLastError());
+        finish_graph(e);
+    });
+}
+
+int lpa_graph_stream(lpa_engine *e, uint64_t *n_c, uint64_t *nbits, uint64_t *words, uint64_t *offsets,
+                     uint32_t *vertices) {
+    return guard([&] {
+        if (!e || !n_c || !nbits) throw Fail(LPA_ERR_ARG, "null argument");
+        if (!e->have_graph) throw Fail(LPA_ERR_STATE, "no graph uploaded");
+        HIPCHK(hipSetDevice(e->device));
+        *n_c = e->n_c;
+        *nbits = e->nbits;
+        if (words && e->nbits)
+            HIPCHK(hipMemcpyAsync(words, e->words0.p, (e->nbits + 63) / 64 * 8, hipMemcpyDeviceToHost, e->stream));
+        if (offsets) std::memcpy(offsets, e->h_offs.data(), (e->n_c + 1) * 8);
+        if (vertices) std::memcpy(vertices, e->h_vertices.data(), e->n * 4);
+        e->sync();
     });
 }
 

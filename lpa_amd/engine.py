@@ -53,6 +53,8 @@ EXPORTS = {
     "lpa_schedule_learned": (C.c_int, [_VP, _VP]),
     "lpa_last_kernel_times": (C.c_int, [_VP, _VP]),
     "lpa_calibration_copy": (C.c_int, [_VP, C.c_uint64, C.c_int, _VP]),
+    "lpa_graph_encode_csr": (C.c_int, [_VP, C.c_uint64, _VP, _VP, C.c_uint32]),
+    "lpa_graph_stream": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP]),
 }
 
 
@@ -123,6 +125,27 @@ class Engine:
     def upload(self, cgr: Cgr):
         self.upload_arrays(cgr.n, cgr.n_c, cgr.words, cgr.nwords, cgr.offsets, cgr.vertices,
                            cgr.small_offsets, cgr.small_nbrs)
+
+    def encode_csr(self, row, col, small_thresh=0):
+        """CGR-encodes a CSR on the device and makes it the engine's graph
+        (the stream is the reference compressor's, bit for bit)"""
+        row = np.ascontiguousarray(row, np.uint64)
+        col = np.ascontiguousarray(col, np.uint32)
+        if len(col) == 0:
+            col = np.zeros(1, np.uint32)
+        n = len(row) - 1
+        _check(self._L.lpa_graph_encode_csr(self._h, n, _p(row), _p(col), small_thresh))
+        self.n = n
+
+    def stream(self):
+        """(words u64 [ceil(nbits/64)], offsets u64 [n_c+1], vertices u32 [n], nbits)"""
+        n_c, nbits = C.c_uint64(0), C.c_uint64(0)
+        _check(self._L.lpa_graph_stream(self._h, C.byref(n_c), C.byref(nbits), None, None, None))
+        words = np.zeros(max((nbits.value + 63) // 64, 1), np.uint64)
+        offs = np.zeros(n_c.value + 1, np.uint64)
+        vert = np.zeros(self.n, np.uint32)
+        _check(self._L.lpa_graph_stream(self._h, C.byref(n_c), C.byref(nbits), _p(words), _p(offs), _p(vert)))
+        return words[:(nbits.value + 63) // 64], offs, vert, nbits.value
 
     def upload_arrays(self, n, n_c, words32, nwords, offsets, vertices, small_offsets=None,
                       small_nbrs=None):

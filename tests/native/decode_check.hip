@@ -1,9 +1,17 @@
 // decode_check.hip -- test library: runs the device CGR decoders of
-// lpa_amd/csrc/hip/cgr_decode.hpp on the host, so the tests can compare
-// them with the CPU oracle without a GPU.
+// lpa_amd/csrc/hip/cgr_decode.hpp and the device record encoder of
+// encode_kernels.hpp on the host, so the tests can compare them with the
+// CPU oracle without a GPU.
 #include <cstdint>
 
 #include "../../lpa_amd/csrc/hip/cgr_decode.hpp"
+#include "../../lpa_amd/csrc/hip/encode_kernels.hpp"
+
+namespace {
+struct HostOr {   // one thread writes the whole stream here
+    static void apply(uint64_t *w, uint64_t v) { *w |= v; }
+};
+}  // namespace
 
 extern "C" {
 
@@ -51,6 +59,22 @@ int64_t dc_decode_segments(const uint64_t *words, uint64_t pos, uint32_t v, uint
         lpa::read_residual_segment(s, v, put);
     }
     return (int64_t)d;
+}
+
+// Bits of vertex v's record (sorted adjacency adj[0..d)), as the device
+// encoder measures it.
+uint64_t dc_encode_bits(uint32_t v, const uint32_t *adj, uint32_t d) {
+    lpa::enc::Meter m{0};
+    lpa::enc::encode_record(m, v, adj, d);
+    return m.pos;
+}
+
+// Writes the record into words (zeroed, 64-bit, bit 0 = MSB of word 0) at
+// bits [start, end).
+void dc_encode_record(uint32_t v, const uint32_t *adj, uint32_t d, uint64_t *words, uint64_t start, uint64_t end) {
+    lpa::enc::BitWriter<HostOr> w(words, start, end);
+    lpa::enc::encode_record(w, v, adj, d);
+    w.finish();
 }
 
 }  // extern "C"
