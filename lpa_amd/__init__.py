@@ -5,6 +5,6 @@ Python binding of the two in-tree native libraries:
   liblpa_hip.so   (include/lpa_amd.h)   gfx950 decode + LP engine
 """
 from . import host  # noqa: F401
-from .host import Cgr, Csr, LpaError, gen_copying, gen_lattice, gen_rmat  # noqa: F401
+from .host import Cgr, Csr, LpaError, gen_copying, gen_lattice, gen_rmat, threads  # noqa: F401
 from .engine import Engine, Mlp, device_count  # noqa: F401,E402
 from . import schedule  # noqa: F401,E402

@@ -194,10 +194,13 @@ constexpr int HEAVY_KB_CMS = 4;
 // sketch s at sk + s*cells).  ADD: add every label to its sketch (LDS,
 // pre-aggregated per wave); else: estimate every label from its sketch and
 // keep the best (count << 32 | ~label) per vector in best[s].
+// relabel (single label vector, decoded list in HBM scratch): the ADD pass
+// writes each gathered label over its id, and the estimating pass reads the
+// labels back in order instead of gathering them again.
 template <bool ADD, int KB>
 __device__ __forceinline__ void cms_rounds(const StepArgs &a, const uint32_t *__restrict__ ids, uint32_t n,
                                            uint32_t k0, uint32_t kb, uint32_t *sk, uint32_t cells,
-                                           uint64_t (&best)[KB]) {
+                                           uint64_t (&best)[KB], bool relabel = false) {
     // the row hashes stay in scalar registers for the whole pass (a runtime
     // row index into the argument block would reload them per label)
     constexpr int MAXD = 8;
@@ -217,7 +220,12 @@ __device__ __forceinline__ void cms_rounds(const StepArgs &a, const uint32_t *__
         for (int t = 0; t < CMS_R; t++) {
             const uint32_t i = i0 + t * HBLOCK + threadIdx.x;
             if (i < n) {
-                load_row<KB>(a, ids[i], k0, kb, labs[t]);
+                if (KB == 1 && relabel && !ADD) {
+                    labs[t][0] = ids[i];
+                } else {
+                    load_row<KB>(a, ids[i], k0, kb, labs[t]);
+                    if (KB == 1 && relabel) const_cast<uint32_t *>(ids)[i] = labs[t][0];
+                }
             } else {
 #pragma unroll
                 for (int q = 0; q < KB; q++) labs[t][q] = UNL;
@@ -322,13 +330,13 @@ __global__ void __launch_bounds__(HBLOCK) k_heavy_chunks(HeavyArgs h) {
                 uint32_t *lsk = (uint32_t *)lds;
                 for (uint32_t x = threadIdx.x; x < kb * cells; x += HBLOCK) lsk[x] = 0;
                 __syncthreads();
-                cms_rounds<true, KB>(a, ids, n, h.k0 + kq, kb, lsk, cells, best);
+                cms_rounds<true, KB>(a, ids, n, h.k0 + kq, kb, lsk, cells, best, KB == 1 && a.K == 1 && !h.csr);
                 __syncthreads();
                 for (uint32_t x = threadIdx.x; x < kb * cells; x += HBLOCK)
                     if (lsk[x]) atomicAdd(&hsk[x], lsk[x]);
                 __syncthreads();
             } else {
-                cms_rounds<false, KB>(a, ids, n, h.k0 + kq, kb, hsk, cells, best);
+                cms_rounds<false, KB>(a, ids, n, h.k0 + kq, kb, hsk, cells, best, KB == 1 && a.K == 1 && !h.csr);
 #pragma unroll
                 for (int q = 0; q < KB; q++) {
                     if ((uint32_t)q >= kb) continue;
@@ -382,9 +390,10 @@ __global__ void __launch_bounds__(HBLOCK, 8) k_heavy_local(HeavyArgs h, const ui
             for (uint32_t x = threadIdx.x; x < kb * cells; x += HBLOCK) sk[x] = 0;
             __syncthreads();
             uint64_t best[KB] = {};
-            cms_rounds<true, KB>(a, ids, d, k0, kb, sk, cells, best);
+            const bool relabel = KB == 1 && a.K == 1 && !h.csr;
+            cms_rounds<true, KB>(a, ids, d, k0, kb, sk, cells, best, relabel);
             __syncthreads();
-            cms_rounds<false, KB>(a, ids, d, k0, kb, sk, cells, best);
+            cms_rounds<false, KB>(a, ids, d, k0, kb, sk, cells, best, relabel);
 #pragma unroll
             for (int q = 0; q < KB; q++) {
                 if ((uint32_t)q >= kb) continue;
