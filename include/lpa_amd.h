@@ -81,7 +81,7 @@ enum {
 };
 
 typedef struct {
-    uint32_t lane_max;      /* deg <= lane_max -> LANE (<= 32) */
+    uint32_t lane_max;      /* deg <= lane_max -> LANE (<= 64) */
     uint32_t wave_max;      /* deg <= wave_max -> WAVE (0: 512) */
     uint32_t block_max;     /* deg <= block_max -> BLOCK, else the heavy-vertex path (0: LDS limit) */
     uint32_t relayout;      /* records are always reordered so each bin is contiguous; keep 1 */

@@ -247,7 +247,8 @@ template <bool CSR> static void launch_lane(int N, StepArgs a, uint32_t count, h
     case 4: hipLaunchKernelGGL((k_lane<4, CSR>), grid, dim3(BLOCK), 0, s, a); break;
     case 8: hipLaunchKernelGGL((k_lane<8, CSR>), grid, dim3(BLOCK), 0, s, a); break;
     case 16: hipLaunchKernelGGL((k_lane<16, CSR>), grid, dim3(BLOCK), 0, s, a); break;
-    default: hipLaunchKernelGGL((k_lane<32, CSR>), grid, dim3(BLOCK), 0, s, a); break;
+    case 32: hipLaunchKernelGGL((k_lane<32, CSR>), grid, dim3(BLOCK), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_lane<64, CSR>), grid, dim3(BLOCK), 0, s, a); break;
     }
 }
 
@@ -889,7 +890,7 @@ static void build_layout(lpa_engine *e, const std::vector<uint8_t> &kind_c, cons
     for (uint64_t i = 0; i < e->n_c; i++) e->kind_of[e->h_vertices[i]] = kind_c[i];
     for (uint64_t j = 0; j < e->n_s; j++) e->kind_of[e->h_vertices[e->n_c + j]] = kind_s[j];
     e->sched_used = p;
-    auto lane_n = [](uint32_t d) { return d <= 4 ? 4 : d <= 8 ? 8 : d <= 16 ? 16 : 32; };
+    auto lane_n = [](uint32_t d) { return d <= 4 ? 4 : d <= 8 ? 8 : d <= 16 ? 16 : d <= 32 ? 32 : 64; };
     auto sub_of = [&](int kind, uint32_t d) -> int {
         switch (kind) {
         case LPA_BIN_LANE: return lane_n(d);
@@ -990,7 +991,7 @@ static uint8_t legal_kind(const lpa_engine *e, int want, uint32_t d, const lpa_s
     if (d == 0) return LPA_BIN_SKIP;
     if (e->cms_thresh && d > e->cms_thresh) return LPA_BIN_CMS;
     // promote to the first kind able to hold the degree
-    if (want <= LPA_BIN_LANE && d <= 32) return LPA_BIN_LANE;
+    if (want <= LPA_BIN_LANE && d <= std::max<uint32_t>(32, std::min<uint32_t>(p.lane_max, 64))) return LPA_BIN_LANE;
     if (want <= LPA_BIN_WAVE && d <= std::min<uint32_t>(p.wave_max ? p.wave_max : wave_cap(e), wave_cap(e))) return LPA_BIN_WAVE;
     if (want <= LPA_BIN_BLOCK && d <= block_cap(e)) return LPA_BIN_BLOCK;
     return LPA_BIN_GLOBAL;
@@ -1000,7 +1001,7 @@ int lpa_schedule_degree(lpa_engine *e, const lpa_sched_params *p) {
     return guard([&] {
         if (!e || !p) throw Fail(LPA_ERR_ARG, "null argument");
         if (!e->have_graph) throw Fail(LPA_ERR_STATE, "no graph uploaded");
-        if (p->lane_max > 32) throw Fail(LPA_ERR_RANGE, "lane_max must be <= 32");
+        if (p->lane_max > 64) throw Fail(LPA_ERR_RANGE, "lane_max must be <= 64");
         HIPCHK(hipSetDevice(e->device));
         lpa_sched_params q = *p;
         if (q.wave_max == 0) q.wave_max = 512;
@@ -1662,7 +1663,7 @@ int lpa_autotune(lpa_engine *e, int iters, lpa_sched_params *best, float *best_m
         float best_t = 1e30f;
         lpa_sched_params bp = e->sched_used;
         std::vector<float> ms(iters);
-        for (uint32_t lane : {8u, 16u, 32u}) {
+        for (uint32_t lane : {8u, 16u, 32u, 64u}) {
             for (uint32_t wave : {64u, 128u, 256u, 512u, 1024u}) {
                 if (wave > wc) continue;
                 for (uint32_t quarter : {64u, 128u}) {

@@ -21,7 +21,8 @@ def test_reference_file_sets(case):
 
 
 @pytest.mark.parametrize("sched", [dict(lane_max=32, wave_max=256), dict(lane_max=0, wave_max=0, block_max=1),
-                                   dict(lane_max=0, wave_max=512), dict(lane_max=8, wave_max=32, block_max=64)])
+                                   dict(lane_max=0, wave_max=512), dict(lane_max=8, wave_max=32, block_max=64),
+                                   dict(lane_max=64, wave_max=512)])
 def test_each_kernel_family(sched):
     g = L.gen_rmat(13, 16, seed=11)
     run_and_compare(g.encode(0, 128), 4, **sched)
@@ -64,6 +65,7 @@ def test_batched_and_semi_supervised():
     lab[seed_mask] = rng.integers(0, 20, seed_mask.sum())
     run_and_compare(cgr, 5, labels0=lab, K=K, semi=True)
     run_and_compare(cgr, 3, K=3)
+    run_and_compare(cgr, 3, K=8, lane_max=64)
 
 
 def test_small_vertex_csr_part():
