@@ -105,6 +105,8 @@ struct HeavySet {
     DevBuf<unsigned long long> result;
     uint32_t ntasks = 0;
     DevBuf<uint2> tasks;                     // decode tasks: (heavy vertex, first segment)
+    DevBuf<uint32_t> stash;                  // CMS, several vectors: label stash per resident workgroup
+    uint32_t stash_blocks = 0;
     DevBuf<unsigned long long> cursor;       // per heavy vertex: list slots claimed by decode tasks
 };
 
@@ -259,10 +261,15 @@ static int wave_groups(uint32_t logT, uint32_t quarter_max = 128) {
     return (1u << (logT - 1)) <= quarter_max ? 16 : 4;
 }
 
-// label vectors staged per row gather by the group kernels (4 keeps the
-// staged registers low enough for four waves per SIMD; the second half of a
-// 32-byte row is an L2 hit when K = 8)
-static int group_kb(uint32_t K) { return K >= 3 ? 4 : K == 2 ? 2 : 1; }
+// label vectors staged per row gather by the group kernels: 4 keeps the
+// staged registers low enough for four waves per SIMD, and while the label
+// rows fit the 256 MB last-level cache the second half of a 32-byte row is
+// a cache hit; beyond it every row gather costs a memory access, and 8
+// vectors per gather (one access per neighbour and step at K = 8) win.
+static int group_kb(uint32_t K, uint64_t n) {
+    const bool cached = (uint64_t)n * K * 4 <= (256ull << 20);
+    return K >= 5 && !cached ? 8 : K >= 3 ? 4 : K == 2 ? 2 : 1;
+}
 
 // instance of the group kernel for a batch size and staged ids per lane
 template <int G, bool CSR> static const void *group_fn(int kb, uint32_t ipl) {
@@ -279,10 +286,11 @@ template <int G, bool CSR> static const void *group_fn(int kb, uint32_t ipl) {
 }
 
 template <int G>
-static void launch_group(bool as_csr, uint32_t K, dim3 grid, size_t lds, hipStream_t s, const StepArgs &a,
-                         uint32_t logT, uint32_t cap) {
+static void launch_group(bool as_csr, uint32_t K, uint64_t n, dim3 grid, size_t lds, hipStream_t s,
+                         const StepArgs &a, uint32_t logT, uint32_t cap) {
     const uint32_t ipl = (cap + G - 1) / G;
-    const void *fn = as_csr ? group_fn<G, true>(group_kb(K), ipl) : group_fn<G, false>(group_kb(K), ipl);
+    const int kb = group_kb(K, n);
+    const void *fn = as_csr ? group_fn<G, true>(kb, ipl) : group_fn<G, false>(kb, ipl);
     StepArgs args = a;
     void *params[] = {&args, &logT, &cap};
     (void)hipLaunchKernel(fn, grid, dim3(BLOCK), params, lds, s);
@@ -311,13 +319,13 @@ void lpa_engine::launch_bin(const Bin &b, StepArgs a, bool as_csr) {
         const int ng = wave_groups(logT, sched_used.quarter_max ? sched_used.quarter_max : 128);
         const size_t lds = group_lds(ng, logT);
         dim3 grid((count + ng - 1) / ng);
-        if (ng == 16) launch_group<16>(as_csr, K, grid, lds, stream, a, logT, cap);
-        else launch_group<64>(as_csr, K, grid, lds, stream, a, logT, cap);
+        if (ng == 16) launch_group<16>(as_csr, K, n, grid, lds, stream, a, logT, cap);
+        else launch_group<64>(as_csr, K, n, grid, lds, stream, a, logT, cap);
         break;
     }
     case LPA_BIN_BLOCK: {
         const uint32_t logT = (uint32_t)b.param, cap = 1u << (logT - 1);
-        launch_group<BLOCK>(as_csr, K, dim3(count), group_lds(1, logT), stream, a, logT, cap);
+        launch_group<BLOCK>(as_csr, K, n, dim3(count), group_lds(1, logT), stream, a, logT, cap);
         break;
     }
     default: break;   // CMS and GLOBAL bins run through launch_heavy
@@ -362,9 +370,16 @@ void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
         const uint32_t nsmall = hs.nlocal - hs.nlocal_big;
         const uint32_t *lsmall = hs.local.p, *lbig = hs.local.p + nsmall;
         if (hs.cms) {
-            const dim3 g(hs.nlocal);
-            if (one) hipLaunchKernelGGL((k_heavy_local<1, 1, 4>), g, b, sk, stream, h, hs.local.p, hs.nlocal);
-            else hipLaunchKernelGGL((k_heavy_local<1, HEAVY_KB_CMS, 4>), g, b, sk, stream, h, hs.local.p, hs.nlocal);
+            if (one) {
+                hipLaunchKernelGGL((k_heavy_local<1, 1, 4>), dim3(hs.nlocal), b, sk, stream, h, hs.local.p, hs.nlocal);
+            } else {
+                // resident workgroups loop over the vertices, each with its stash
+                h.stash = hs.stash.p;
+                h.stash_stride = (uint64_t)CMS_LOCAL * HEAVY_KB_CMS;
+                const uint32_t g = hs.stash_blocks ? std::min(hs.nlocal, hs.stash_blocks) : hs.nlocal;
+                hipLaunchKernelGGL((k_heavy_local<1, HEAVY_KB_CMS, 4>), dim3(g), b, sk, stream, h, hs.local.p,
+                                   hs.nlocal);
+            }
         } else {
             // tables of up to 2*CHUNK (8192 ids: 4 per lane) or 4*CHUNK slots (8 per lane)
             const size_t t1 = 2 * CHUNK * sizeof(uint64_t), t2 = 4 * CHUNK * sizeof(uint64_t);
@@ -875,6 +890,17 @@ static void build_heavy(lpa_engine *e) {
                 hs->tab.alloc(std::max<uint64_t>(toff.back(), 1) * hs->kb);
             }
             hs->result.alloc((uint64_t)std::max<uint32_t>(hs->nchunked, 1) * hs->kb);
+            if (hs->cms && !hs->csr_part && hs->nlocal && e->K > 1) {
+                // one stash of CMS_LOCAL rows of HEAVY_KB_CMS labels per workgroup the CUs hold at once
+                const size_t cells = (size_t)e->cms_depth * e->cms_width;
+                const size_t kbs = std::max<size_t>(1, std::min<size_t>({(size_t)e->K, (size_t)HEAVY_KB_CMS,
+                                                                          (size_t)lds_budget(e) / (cells * 4)}));
+                int per_cu = 0;
+                HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &per_cu, (const void *)k_heavy_local<1, HEAVY_KB_CMS, 4>, HBLOCK, kbs * cells * 4));
+                hs->stash_blocks = (uint32_t)std::min<uint64_t>(hs->nlocal, (uint64_t)std::max(per_cu, 1) * e->num_cu);
+                hs->stash.alloc((uint64_t)hs->stash_blocks * CMS_LOCAL * HEAVY_KB_CMS);
+            }
             e->heavy.push_back(std::move(hs));
         }
     }

@@ -54,6 +54,8 @@ struct HeavyArgs {
     uint32_t k0, kb;                           // label vectors of this batch
     uint32_t kbs;                              // CMS: label vectors per sketch pass (sketches side by side in LDS)
     const uint32_t *__restrict__ slot;         // chunked vertex i -> its sketch / result slot
+    uint32_t *stash;                           // CMS, several vectors: per-workgroup label stash (or NULL)
+    uint64_t stash_stride;                     // words per workgroup
 };
 
 __device__ __forceinline__ const uint32_t *heavy_ids(const HeavyArgs &h, uint32_t i) {
@@ -194,13 +196,15 @@ constexpr int HEAVY_KB_CMS = 4;
 // sketch s at sk + s*cells).  ADD: add every label to its sketch (LDS,
 // pre-aggregated per wave); else: estimate every label from its sketch and
 // keep the best (count << 32 | ~label) per vector in best[s].
-// relabel (single label vector, decoded list in HBM scratch): the ADD pass
-// writes each gathered label over its id, and the estimating pass reads the
-// labels back in order instead of gathering them again.
+// stash (HBM, n*KB words, or NULL): the ADD pass stores every gathered
+// label there ([id][KB]) and the estimating pass reads the labels back in
+// order instead of gathering the rows again (a second random gather of a
+// graph whose labels do not fit the caches costs a full memory access per
+// arc).  With one label vector the decoded list itself is the stash.
 template <bool ADD, int KB>
 __device__ __forceinline__ void cms_rounds(const StepArgs &a, const uint32_t *__restrict__ ids, uint32_t n,
                                            uint32_t k0, uint32_t kb, uint32_t *sk, uint32_t cells,
-                                           uint64_t (&best)[KB], bool relabel = false) {
+                                           uint64_t (&best)[KB], uint32_t *stash = nullptr) {
     // the row hashes stay in scalar registers for the whole pass (a runtime
     // row index into the argument block would reload them per label)
     constexpr int MAXD = 8;
@@ -220,11 +224,29 @@ __device__ __forceinline__ void cms_rounds(const StepArgs &a, const uint32_t *__
         for (int t = 0; t < CMS_R; t++) {
             const uint32_t i = i0 + t * HBLOCK + threadIdx.x;
             if (i < n) {
-                if (KB == 1 && relabel && !ADD) {
-                    labs[t][0] = ids[i];
+                if (stash && !ADD) {
+                    if (KB == 4) {
+                        const uint4 w = reinterpret_cast<const uint4 *>(stash)[i];
+                        labs[t][0] = w.x;
+                        labs[t][KB > 1 ? 1 : 0] = w.y;
+                        labs[t][KB > 2 ? 2 : 0] = w.z;
+                        labs[t][KB > 3 ? 3 : 0] = w.w;
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < KB; q++) labs[t][q] = stash[(uint64_t)i * KB + q];
+                    }
                 } else {
                     load_row<KB>(a, ids[i], k0, kb, labs[t]);
-                    if (KB == 1 && relabel) const_cast<uint32_t *>(ids)[i] = labs[t][0];
+                    if (stash) {
+                        if (KB == 4) {
+                            reinterpret_cast<uint4 *>(stash)[i] =
+                                make_uint4(labs[t][0], labs[t][KB > 1 ? 1 : 0], labs[t][KB > 2 ? 2 : 0],
+                                           labs[t][KB > 3 ? 3 : 0]);
+                        } else {
+#pragma unroll
+                            for (int q = 0; q < KB; q++) stash[(uint64_t)i * KB + q] = labs[t][q];
+                        }
+                    }
                 }
             } else {
 #pragma unroll
@@ -330,13 +352,15 @@ __global__ void __launch_bounds__(HBLOCK) k_heavy_chunks(HeavyArgs h) {
                 uint32_t *lsk = (uint32_t *)lds;
                 for (uint32_t x = threadIdx.x; x < kb * cells; x += HBLOCK) lsk[x] = 0;
                 __syncthreads();
-                cms_rounds<true, KB>(a, ids, n, h.k0 + kq, kb, lsk, cells, best, KB == 1 && a.K == 1 && !h.csr);
+                cms_rounds<true, KB>(a, ids, n, h.k0 + kq, kb, lsk, cells, best,
+                                     KB == 1 && a.K == 1 && !h.csr ? const_cast<uint32_t *>(ids) : nullptr);
                 __syncthreads();
                 for (uint32_t x = threadIdx.x; x < kb * cells; x += HBLOCK)
                     if (lsk[x]) atomicAdd(&hsk[x], lsk[x]);
                 __syncthreads();
             } else {
-                cms_rounds<false, KB>(a, ids, n, h.k0 + kq, kb, hsk, cells, best, KB == 1 && a.K == 1 && !h.csr);
+                cms_rounds<false, KB>(a, ids, n, h.k0 + kq, kb, hsk, cells, best,
+                                      KB == 1 && a.K == 1 && !h.csr ? const_cast<uint32_t *>(ids) : nullptr);
 #pragma unroll
                 for (int q = 0; q < KB; q++) {
                     if ((uint32_t)q >= kb) continue;
@@ -353,21 +377,8 @@ __global__ void __launch_bounds__(HBLOCK) k_heavy_chunks(HeavyArgs h) {
     }
 }
 
-// Heavy vertices small enough for one workgroup's LDS (exact: degree <=
-// 2*CHUNK, table of 2^ceil(log2 2d) slots -- up to 128 KB; CMS: any degree
-// up to the local limit, kbs sketches in LDS): one workgroup per vertex
-// counts its whole list -- both CMS passes, or the exact table -- and writes
-// the labels itself, with no HBM table, sketch or merge.  local[] lists the
-// heavy-vertex indices; IPL = staged ids per lane (exact).
-// (Eight waves per SIMD requested: two 1024-lane workgroups per CU when their
-// tables fit, at the price of a few spilled registers in the widest case.)
 template <int CMS, int KB, int IPL>
-__global__ void __launch_bounds__(HBLOCK, 8) k_heavy_local(HeavyArgs h, const uint32_t *__restrict__ local,
-                                                       uint32_t nlocal) {
-    extern __shared__ uint64_t lds[];
-    __shared__ uint64_t red[HBLOCK / 64];
-    if (blockIdx.x >= nlocal) return;
-    const uint32_t i = local[blockIdx.x];
+__device__ __forceinline__ void heavy_local_vertex(const HeavyArgs &h, uint32_t i, uint64_t *lds, uint64_t *red) {
     const uint32_t d = heavy_deg(h, i);
     const uint32_t *ids = heavy_ids(h, i);
     const StepArgs &a = h.s;
@@ -390,10 +401,14 @@ __global__ void __launch_bounds__(HBLOCK, 8) k_heavy_local(HeavyArgs h, const ui
             for (uint32_t x = threadIdx.x; x < kb * cells; x += HBLOCK) sk[x] = 0;
             __syncthreads();
             uint64_t best[KB] = {};
-            const bool relabel = KB == 1 && a.K == 1 && !h.csr;
-            cms_rounds<true, KB>(a, ids, d, k0, kb, sk, cells, best, relabel);
+            // stash: the decoded list itself for one vector, else this
+            // workgroup's region of the stash buffer
+            uint32_t *stash = h.csr ? nullptr
+                            : KB == 1 && a.K == 1 ? const_cast<uint32_t *>(ids)
+                            : h.stash ? h.stash + (uint64_t)blockIdx.x * h.stash_stride : nullptr;
+            cms_rounds<true, KB>(a, ids, d, k0, kb, sk, cells, best, stash);
             __syncthreads();
-            cms_rounds<false, KB>(a, ids, d, k0, kb, sk, cells, best, relabel);
+            cms_rounds<false, KB>(a, ids, d, k0, kb, sk, cells, best, stash);
 #pragma unroll
             for (int q = 0; q < KB; q++) {
                 if ((uint32_t)q >= kb) continue;
@@ -435,6 +450,29 @@ __global__ void __launch_bounds__(HBLOCK, 8) k_heavy_local(HeavyArgs h, const ui
     }
     if (threadIdx.x == 0 && nch) atomicAdd(a.changed, (unsigned long long)nch);
     if (a.rec && threadIdx.x == 0) atomicAdd(&a.rec[v], (cycles() - t0) * HBLOCK);
+}
+
+// Heavy vertices small enough for one workgroup's LDS (exact: degree <=
+// 2*CHUNK, table of 2^ceil(log2 2d) slots -- up to 128 KB; CMS: any degree
+// up to the local limit, kbs sketches in LDS): one workgroup per vertex
+// counts its whole list -- both CMS passes, or the exact table -- and writes
+// the labels itself, with no HBM table, sketch or merge.  local[] lists the
+// heavy-vertex indices; IPL = staged ids per lane (exact).
+// (Eight waves per SIMD requested: two 1024-lane workgroups per CU when their
+// tables fit, at the price of a few spilled registers in the widest case.)
+template <int CMS, int KB, int IPL>
+__global__ void __launch_bounds__(HBLOCK, 8) k_heavy_local(HeavyArgs h, const uint32_t *__restrict__ local,
+                                                       uint32_t nlocal) {
+    extern __shared__ uint64_t lds[];
+    __shared__ uint64_t red[HBLOCK / 64];
+    if (CMS) {
+        // workgroups loop over the vertices (the grid may be smaller than
+        // the list when each workgroup owns a stash region)
+        for (uint32_t j = blockIdx.x; j < nlocal; j += gridDim.x)
+            heavy_local_vertex<CMS, KB, IPL>(h, local[j], lds, red);
+    } else if (blockIdx.x < nlocal) {
+        heavy_local_vertex<CMS, KB, IPL>(h, local[blockIdx.x], lds, red);
+    }
 }
 
 template <int CMS>

@@ -38,6 +38,13 @@ def test_rmat22_two_steps_k8(rmat22):
     run_and_compare(rmat22, 2, K=8, **CMS22)
 
 
+def test_rmat20_k64_uncached_rows():
+    """64 label vectors of a million vertices (268 MB of label rows, more
+    than the last-level cache): the kernels gather 8 vectors per row"""
+    g = L.gen_rmat(20, 16, 0.57, 0.19, 0.19, seed=6, bfs=True)
+    run_and_compare(g.encode(0, 128), 2, K=64, cms_thresh=3000, cms_depth=4, cms_width=1024, cms_seed=4)
+
+
 def one_step_checks(cgr, iters, check_at, K, semi=False, labels0=None, sample=1_000_000, **cms):
     """runs `iters` engine steps; after each step in check_at, recomputes a
     sample of vertices with the oracle from the engine's previous labels"""
