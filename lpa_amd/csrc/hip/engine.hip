@@ -1606,15 +1606,26 @@ int lpa_schedule_learned(lpa_engine *e, lpa_mlp *m) {
         HIPCHK(hipSetDevice(e->device));
         std::vector<uint32_t> deg;
         degree_of_vertex(e, deg);
-        lpa_sched_params q{32, 0, 0, 1};
+        lpa_sched_params q{32, 512, 0, 1, 128};
         const uint32_t wc = wave_cap(e), bc = block_cap(e);
         std::vector<uint8_t> byv(e->n);
+        // A vertex leaves the family the default degree schedule gives it
+        // only when the model predicts another one cheaper by more than
+        // LEARNED_MARGIN (log2 of the cost): near-ties stay put, so noise in
+        // the recorded costs does not reshuffle the bins from run to run.
+        // Family 0 stops at the default wave limit: above it the wave
+        // kernel's lists outgrow its staged registers, which the recorded
+        // costs (every vertex of the graph in one family) understate.
+        constexpr float LEARNED_MARGIN = 0.5f;   // ~41 %
+        const uint32_t def_wave = std::min<uint32_t>(512, wc);
         for (uint64_t v = 0; v < e->n; v++) {
             const uint32_t d = deg[v];
             int fam = 2;
             float best = y[v * MLP_OUT + 2];
             if (d <= bc && y[v * MLP_OUT + 1] < best) { fam = 1; best = y[v * MLP_OUT + 1]; }
-            if (d <= wc && y[v * MLP_OUT + 0] < best) { fam = 0; best = y[v * MLP_OUT + 0]; }
+            if (d <= def_wave && y[v * MLP_OUT + 0] < best) { fam = 0; best = y[v * MLP_OUT + 0]; }
+            const int def = d <= def_wave ? 0 : d <= bc ? 1 : 2;
+            if (fam != def && y[v * MLP_OUT + def] - best < LEARNED_MARGIN) fam = def;
             const int want = fam == 0 ? LPA_BIN_LANE : fam == 1 ? LPA_BIN_BLOCK : LPA_BIN_GLOBAL;
             byv[v] = legal_kind(e, want, d, q);
         }
