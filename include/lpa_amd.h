@@ -202,7 +202,8 @@ int lpa_mlp_train(lpa_mlp *m, const float *x, const float *target, const uint8_t
  * families that can hold its degree; CMS vertices stay CMS) and lays the
  * graph out accordingly.  A vertex leaves the family of the default degree
  * schedule only when another is predicted cheaper by more than ~41 %, and
- * the wave family is offered up to degree 512. */
+ * the wave family is offered up to degree 512; vertices of the small CSR
+ * part keep their degree family. */
 int lpa_schedule_learned(lpa_engine *e, lpa_mlp *m);
 
 /* Runs iters steps (method 0 = NOVA fused, 1 = CGLP) and reports the

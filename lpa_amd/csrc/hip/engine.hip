@@ -104,10 +104,9 @@ struct HeavySet {
     DevBuf<uint32_t> sketch;
     DevBuf<unsigned long long> result;
     uint32_t ntasks = 0;
-    DevBuf<uint2> tasks;                     // decode tasks: (heavy vertex, first segment)
+    DevBuf<DecodeTask> tasks;                // decode tasks (fixed per layout)
     DevBuf<uint32_t> stash;                  // CMS, several vectors: label stash per resident workgroup
     uint32_t stash_blocks = 0;
-    DevBuf<unsigned long long> cursor;       // per heavy vertex: list slots claimed by decode tasks
 };
 
 struct Bin {
@@ -145,6 +144,7 @@ struct lpa_engine {
     DevBuf<uint32_t> vid0;
     DevBuf<uint4> meta0;                    // segment metadata per file-order compressed position
     std::vector<uint64_t> h_offs;           // file-order start offsets
+    std::vector<uint64_t> h_loffs;          // start offsets of the scheduled (relayouted) stream
     std::vector<uint32_t> h_vertices;       // file order
     std::vector<uint64_t> h_soffs;          // small CSR (file order)
     std::vector<uint32_t> h_snbr;
@@ -289,7 +289,9 @@ template <int G>
 static void launch_group(bool as_csr, uint32_t K, uint64_t n, dim3 grid, size_t lds, hipStream_t s,
                          const StepArgs &a, uint32_t logT, uint32_t cap) {
     const uint32_t ipl = (cap + G - 1) / G;
-    const int kb = group_kb(K, n);
+    // quarter-wave and wave groups hold few labels per lane: all 8 vectors
+    // of a batch fit their registers, and one gather per row wins
+    const int kb = G <= 64 && K >= 5 ? 8 : group_kb(K, n);
     const void *fn = as_csr ? group_fn<G, true>(kb, ipl) : group_fn<G, false>(kb, ipl);
     StepArgs args = a;
     void *params[] = {&args, &logT, &cap};
@@ -356,13 +358,8 @@ void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
     h.kbs = (uint32_t)std::max<size_t>(
         1, std::min<size_t>({(size_t)K, (size_t)(one ? 1 : HEAVY_KB_CMS), (size_t)(lds_max - 20480) / (cells * 4)}));
     if (!as_csr) {
-        hipLaunchKernelGGL(k_fill_u64, dim3((hs.nh + 255) / 256), dim3(256), 0, stream, hs.cursor.p, (uint64_t)hs.nh, 0ull);
-        if (hs.cms)
-            hipLaunchKernelGGL(k_heavy_decode<1>, dim3(hs.ntasks), dim3(BLOCK), 0, stream, h, hs.tasks.p, hs.ntasks,
-                               hs.cursor.p);
-        else
-            hipLaunchKernelGGL(k_heavy_decode<0>, dim3(hs.ntasks), dim3(BLOCK), 0, stream, h, hs.tasks.p, hs.ntasks,
-                               hs.cursor.p);
+        if (hs.cms) hipLaunchKernelGGL(k_heavy_decode<1>, dim3(hs.ntasks), dim3(BLOCK), 0, stream, h, hs.tasks.p, hs.ntasks);
+        else hipLaunchKernelGGL(k_heavy_decode<0>, dim3(hs.ntasks), dim3(BLOCK), 0, stream, h, hs.tasks.p, hs.ntasks);
     }
     if (hs.nlocal) {
         const dim3 b(HBLOCK);
@@ -863,8 +860,8 @@ static void build_heavy(lpa_engine *e) {
             hs->chunk_b.upload(cb.data(), cb.size(), e->stream);
             if (!hs->csr_part) {
                 hs->list.alloc(loff.back());
-                hs->cursor.alloc(hs->nh);
-                // segments per record (device metadata) -> decode tasks of <= BLOCK segments
+                // decode tasks of <= BLOCK segments per record, with their
+                // seek positions (device metadata) and output ranges
                 DevBuf<uint4> m;
                 m.alloc(hs->nh);
                 hipLaunchKernelGGL(k_gather_u4, dim3((hs->nh + 255) / 256), dim3(256), 0, e->stream, e->meta.p,
@@ -872,13 +869,46 @@ static void build_heavy(lpa_engine *e) {
                 std::vector<uint4> hm(hs->nh);
                 HIPCHK(hipMemcpyAsync(hm.data(), m.p, hs->nh * sizeof(uint4), hipMemcpyDeviceToHost, e->stream));
                 e->sync();
-                std::vector<uint2> tk;
+                std::vector<DecodeTask> tk;
+                std::vector<uint32_t> owner;
                 for (uint32_t i = 0; i < hs->nh; i++) {
+                    const uint32_t q = pos[i];
+                    const uint64_t start = e->h_loffs[q];   // the scheduled stream
                     const uint32_t segs = hm[i].y + hm[i].w;
-                    for (uint32_t j0 = 0; j0 < segs; j0 += BLOCK) tk.push_back(make_uint2(i, j0));
+                    for (uint32_t j0 = 0; j0 < segs; j0 += BLOCK) {
+                        DecodeTask t{};
+                        t.ibase = start + hm[i].x;
+                        t.rbase = start + hm[i].z;
+                        t.v = e->h_vertices[e->order_c[q]];
+                        t.j0 = j0;
+                        t.ni = hm[i].y;
+                        t.ns = segs;
+                        tk.push_back(t);
+                        owner.push_back(i);
+                    }
                 }
                 hs->ntasks = (uint32_t)tk.size();
                 hs->tasks.upload(tk.data(), tk.size(), e->stream);
+                if (hs->ntasks) {
+                    StepArgs a = e->base_args(false);
+                    hipLaunchKernelGGL(k_heavy_task_count, dim3(hs->ntasks), dim3(BLOCK), 0, e->stream, a, hs->tasks.p,
+                                       hs->ntasks);
+                    HIPCHK(hipGetLastError());
+                    HIPCHK(hipMemcpyAsync(tk.data(), hs->tasks.p, tk.size() * sizeof(DecodeTask), hipMemcpyDeviceToHost,
+                                          e->stream));
+                    e->sync();
+                    // tasks of one vertex fill its list back to back
+                    for (size_t t = 0; t < tk.size(); t++) {
+                        const uint32_t i = owner[t];
+                        tk[t].out = (t > 0 && owner[t - 1] == i) ? tk[t - 1].out + tk[t - 1].count : loff[i];
+                    }
+                    for (uint32_t i = 0, t = 0; i < hs->nh; i++) {
+                        uint64_t sum = 0;
+                        for (; t < tk.size() && owner[t] == i; t++) sum += tk[t].count;
+                        if (sum != loff[i + 1] - loff[i]) throw Fail(LPA_ERR_FORMAT, "heavy record decodes to a wrong degree");
+                    }
+                    hs->tasks.upload(tk.data(), tk.size(), e->stream);
+                }
             }
             if (hs->cms) {
                 hs->kb = e->K;
@@ -962,6 +992,7 @@ static void build_layout(lpa_engine *e, const std::vector<uint8_t> &kind_c, cons
             e->words.alloc(n64);
             HIPCHK(hipMemsetAsync(e->words.p, 0, n64 * 8, e->stream));
             e->offs.upload(noffs.data(), cnt + 1, e->stream);
+            e->h_loffs = noffs;
             e->vid.alloc(std::max<uint64_t>(cnt, 1));
             e->meta.alloc(std::max<uint64_t>(cnt, 1));
             DevBuf<uint32_t> perm;
@@ -1628,6 +1659,12 @@ int lpa_schedule_learned(lpa_engine *e, lpa_mlp *m) {
             if (fam != def && y[v * MLP_OUT + def] - best < LEARNED_MARGIN) fam = def;
             const int want = fam == 0 ? LPA_BIN_LANE : fam == 1 ? LPA_BIN_BLOCK : LPA_BIN_GLOBAL;
             byv[v] = legal_kind(e, want, d, q);
+        }
+        // the small CSR part has no CGR features to predict from: its
+        // vertices keep the default degree family
+        for (uint64_t j = e->n_c; j < e->n; j++) {
+            const uint32_t v = e->h_vertices[j], d = deg[v];
+            byv[v] = legal_kind(e, d <= 32 ? LPA_BIN_LANE : d <= def_wave ? LPA_BIN_WAVE : LPA_BIN_BLOCK, d, q);
         }
         std::vector<uint8_t> kc, ks;
         split_kinds(e, byv, kc, ks);

@@ -66,78 +66,90 @@ __device__ __forceinline__ uint32_t heavy_deg(const HeavyArgs &h, uint32_t i) {
     return (uint32_t)(h.loff[i + 1] - h.loff[i]);
 }
 
-// One workgroup per decode task: up to BLOCK consecutive segments (one per
-// lane; intervals first, then residuals) of heavy vertex tasks[t].x starting
-// at segment tasks[t].y, so a hub's record is decoded by many workgroups at
-// once.  The order of a list is irrelevant to the counts, so a task takes
-// its output range from the vertex's cursor (zero before the step) with one
-// atomic: lanes first count their segments' ids (pass A), then decode again
-// into their part of the range (pass B); runs of LONG_RUN or more ids are
-// written by the whole workgroup.  A task's ids are assembled in LDS and
-// copied out with coalesced stores (one lane per id), unless they exceed
-// the staging buffer.
+// Decode tasks: up to BLOCK consecutive segments (one per lane; intervals
+// first, then residuals) of one heavy record, so a hub's record is decoded
+// by many workgroups at once.  A task's descriptor is fixed for a layout:
+// where its record's interval and residual segments start, the vertex, its
+// first segment, and where its ids go in the vertex's list and how many
+// there are (counted once, by k_heavy_task_count, when the layout is built;
+// the order of a list is irrelevant to the counts).
+struct DecodeTask {
+    uint64_t ibase, rbase;   // absolute bit of interval segment 0 / residual segment 0
+    uint64_t out;            // first list slot of the task's ids
+    uint32_t v, j0, ni, ns;  // vertex, first segment, #interval segments, #segments
+    uint32_t count, pad;     // ids the task writes
+};
 constexpr uint32_t DECODE_STAGE = 8192;
 
+__device__ __forceinline__ uint32_t task_segment(const StepArgs &a, const DecodeTask &T, uint32_t j, BitReader &r) {
+    r.seek(a.words, j < T.ni ? T.ibase + (uint64_t)SEG_BITS * j : T.rbase + (uint64_t)SEG_BITS * (j - T.ni));
+    return j < T.ni;
+}
+
+// One-time: ids of every task (pass of the decoder that only counts).
+__global__ void __launch_bounds__(BLOCK) k_heavy_task_count(StepArgs a, DecodeTask *tasks, uint32_t ntasks) {
+    __shared__ uint32_t total;
+    const uint32_t t = blockIdx.x;
+    if (t >= ntasks) return;
+    const DecodeTask T = tasks[t];
+    if (threadIdx.x == 0) total = 0;
+    __syncthreads();
+    const uint32_t j = T.j0 + threadIdx.x;
+    uint32_t mine = 0;
+    if (j < T.ns) {
+        BitReader r;
+        if (task_segment(a, T, j, r)) read_interval_segment(r, T.v, [&](uint32_t, uint32_t len) { mine += len; });
+        else mine = r.gamma();
+    }
+    if (mine) atomicAdd(&total, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) tasks[t].count = total;
+}
+
+// One workgroup per task: each lane decodes its segment, claiming output
+// slots with one LDS atomic per run (runs of LONG_RUN or more ids are
+// written by the whole workgroup) or per residual segment; the task's ids
+// are assembled in LDS and stored to the list with coalesced stores, or
+// written in place when they exceed the staging buffer.
 // CMS: the heavy set's mode (names the launch in profiles; same code)
 template <int CMS>
-__global__ void __launch_bounds__(BLOCK) k_heavy_decode(HeavyArgs h, const uint2 *__restrict__ tasks,
-                                                        uint32_t ntasks, unsigned long long *__restrict__ cursor) {
+__global__ void __launch_bounds__(BLOCK) k_heavy_decode(HeavyArgs h, const DecodeTask *__restrict__ tasks,
+                                                        uint32_t ntasks) {
     __shared__ GroupShared<BLOCK> sh;
-    __shared__ uint64_t base;
     __shared__ uint32_t stage[DECODE_STAGE];
     const uint32_t t = blockIdx.x;
     if (t >= ntasks || h.csr) return;
     const uint64_t t0 = h.s.rec ? cycles() : 0;
-    const uint32_t i = tasks[t].x, j = tasks[t].y + threadIdx.x;
-    const uint32_t p = h.pos[i];
-    const uint32_t v = h.s.vid[p];
-    const uint64_t start = h.s.offs[p];
-    const uint4 m = h.s.meta[p];
-    const uint32_t ni = m.y, ns = m.y + m.w;
+    const DecodeTask T = tasks[t];
     if (threadIdx.x == 0) {
         sh.cnt = 0;
         sh.qn = 0;
     }
     __syncthreads();
-    BitReader r;
-    uint32_t mine = 0;
-    if (j < ns) {
-        if (j < ni) {
-            r.seek(h.s.words, start + m.x + (uint64_t)SEG_BITS * j);
-            BitReader q = r;
-            read_interval_segment(q, v, [&](uint32_t, uint32_t len) { mine += len; });
-        } else {
-            r.seek(h.s.words, start + m.z + (uint64_t)SEG_BITS * (j - ni));
-            BitReader q = r;
-            mine = q.gamma();
-        }
-    }
-    uint32_t o = mine ? atomicAdd(&sh.cnt, mine) : 0;
-    __syncthreads();
-    const uint32_t total = sh.cnt;
-    if (threadIdx.x == 0) base = atomicAdd(&cursor[i], (unsigned long long)total);
-    __syncthreads();
-    uint32_t *out = h.list + h.loff[i] + base;
-    const bool staged = total <= DECODE_STAGE;
+    uint32_t *out = h.list + T.out;
+    const bool staged = T.count <= DECODE_STAGE;
     uint32_t *dst = staged ? stage : out;
-    if (j < ns) {
-        if (j < ni) {
-            read_interval_segment(r, v, [&](uint32_t l, uint32_t len) {
+    const uint32_t j = T.j0 + threadIdx.x;
+    if (j < T.ns) {
+        BitReader r;
+        if (task_segment(h.s, T, j, r)) {
+            read_interval_segment(r, T.v, [&](uint32_t l, uint32_t len) {
+                const uint32_t o = atomicAdd(&sh.cnt, len);
                 if (len >= LONG_RUN) {
                     const uint32_t q = atomicAdd(&sh.qn, 1u);
                     if (q < (uint32_t)GroupShared<BLOCK>::QCAP) {
                         sh.qleft[q] = l;
                         sh.qlen[q] = len;
                         sh.qpos[q] = o;
-                        o += len;
                         return;
                     }
                 }
                 for (uint32_t x = 0; x < len; x++) dst[o + x] = l + x;
-                o += len;
             });
         } else {
-            read_residual_segment(r, v, [&](uint32_t u) { dst[o++] = u; });
+            BitReader peek = r;   // the segment's count claims its slots
+            uint32_t o = atomicAdd(&sh.cnt, peek.gamma());
+            read_residual_segment(r, T.v, [&](uint32_t u) { dst[o++] = u; });
         }
     }
     __syncthreads();
@@ -148,9 +160,9 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_decode(HeavyArgs h, const uint2
     }
     if (staged) {
         __syncthreads();
-        for (uint32_t x = threadIdx.x; x < total; x += BLOCK) out[x] = stage[x];
+        for (uint32_t x = threadIdx.x; x < T.count; x += BLOCK) out[x] = stage[x];
     }
-    if (h.s.rec && threadIdx.x == 0) atomicAdd(&h.s.rec[v], (cycles() - t0) * BLOCK);
+    if (h.s.rec && threadIdx.x == 0) atomicAdd(&h.s.rec[T.v], (cycles() - t0) * BLOCK);
 }
 
 // chunked[] lists the heavy-vertex indices handled by the chunk kernels.
