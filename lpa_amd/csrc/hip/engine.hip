@@ -318,10 +318,15 @@ void lpa_engine::launch_bin(const Bin &b, StepArgs a, bool as_csr) {
     case LPA_BIN_WAVE: {
         // a quarter wave per vertex up to degree 128, a whole wave above
         const uint32_t logT = (uint32_t)b.param, cap = 1u << (logT - 1);
-        const int ng = wave_groups(logT, sched_used.quarter_max ? sched_used.quarter_max : 128);
+        int ng = wave_groups(logT, sched_used.quarter_max ? sched_used.quarter_max : 128);
+        // with 8 vectors per staged gather, lists of 65..128 ids take half
+        // waves (4 ids per lane) rather than quarter waves (8 per lane):
+        // half the staged registers, twice the waves resident
+        if (ng == 16 && cap > 64 && K >= 5) ng = 8;
         const size_t lds = group_lds(ng, logT);
         dim3 grid((count + ng - 1) / ng);
         if (ng == 16) launch_group<16>(as_csr, K, n, grid, lds, stream, a, logT, cap);
+        else if (ng == 8) launch_group<32>(as_csr, K, n, grid, lds, stream, a, logT, cap);
         else launch_group<64>(as_csr, K, n, grid, lds, stream, a, logT, cap);
         break;
     }
@@ -557,6 +562,7 @@ int lpa_engine_create(int device, lpa_engine **out) {
             HIPCHK(hipHostMalloc((void **)&e->h_counter, sizeof(unsigned long long), hipHostMallocDefault));
             const int dyn = (int)(e->lds_max - 20480);
             allow_group_lds<16>(dyn);
+            allow_group_lds<32>(dyn);
             allow_group_lds<64>(dyn);
             allow_group_lds<BLOCK>(dyn);
             const void *heavy_fns[] = {(const void *)k_heavy_chunks<0, 1>, (const void *)k_heavy_chunks<0, HEAVY_KB_MAX>,

@@ -288,7 +288,7 @@ __device__ __forceinline__ uint32_t cms_col(const StepArgs &a, uint32_t r, uint3
 constexpr uint32_t LONG_RUN = 16;
 
 template <int G> struct GroupShared {
-    static constexpr int QCAP = G == BLOCK ? 512 : G == 64 ? 64 : 16;
+    static constexpr int QCAP = G == BLOCK ? 512 : G == 64 ? 64 : G == 32 ? 32 : 16;
     uint64_t res_start;
     uint32_t cnt;                 // ids written to the list so far
     uint32_t qn;
