@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -107,6 +108,8 @@ struct HeavySet {
     DevBuf<DecodeTask> tasks;                // decode tasks (fixed per layout)
     DevBuf<uint32_t> stash;                  // CMS, several vectors: label stash per resident workgroup
     uint32_t stash_blocks = 0;
+    bool tagged = false;                     // exact: generation-tagged tables (see TABLE_GENS)
+    DevBuf<uint32_t> epoch;                  // exact: passes completed
 };
 
 struct Bin {
@@ -339,6 +342,19 @@ void lpa_engine::launch_bin(const Bin &b, StepArgs a, bool as_csr) {
     }
 }
 
+// One-workgroup CMS kernel: label vectors per pass and its LDS bytes.  One
+// vector keeps 32-bit cells; several take 16-bit cells, two vectors to a
+// word, as many pairs as the LDS holds (at most HEAVY_KB_CMS_LOCAL vectors).
+static uint32_t cms_local_kb(uint32_t K, size_t cells, size_t lds) {
+    if (K == 1) return 1;
+    const size_t pairs = std::max<size_t>(1, lds / (cells * sizeof(uint32_t)));
+    return (uint32_t)std::min<size_t>({(size_t)K, (size_t)HEAVY_KB_CMS_LOCAL, 2 * pairs});
+}
+
+static size_t cms_local_lds(uint32_t kbl, size_t cells) {
+    return (kbl == 1 ? 1 : (kbl + 1) / 2) * cells * sizeof(uint32_t);
+}
+
 void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
     if (hs.nh == 0) return;
     HeavyArgs h{};
@@ -357,18 +373,22 @@ void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
     h.chunk_b = hs.chunk_b.p;
     h.cms = hs.cms ? 1 : 0;
     h.slot = hs.slot.p;
+    h.epoch = hs.epoch.p;
+    h.tagged = hs.tagged ? 1 : 0;
+    h.kb_alloc = hs.kb;
     // CMS: sketches of up to KB label vectors side by side in LDS
     const bool one = K == 1;
     const size_t cells = (size_t)cms_depth * cms_width;
     h.kbs = (uint32_t)std::max<size_t>(
         1, std::min<size_t>({(size_t)K, (size_t)(one ? 1 : HEAVY_KB_CMS), (size_t)(lds_max - 20480) / (cells * 4)}));
+    h.kbl = cms_local_kb(K, cells, lds_max - 20480);
     if (!as_csr) {
         if (hs.cms) hipLaunchKernelGGL(k_heavy_decode<1>, dim3(hs.ntasks), dim3(BLOCK), 0, stream, h, hs.tasks.p, hs.ntasks);
         else hipLaunchKernelGGL(k_heavy_decode<0>, dim3(hs.ntasks), dim3(BLOCK), 0, stream, h, hs.tasks.p, hs.ntasks);
     }
     if (hs.nlocal) {
         const dim3 b(HBLOCK);
-        const size_t sk = h.kbs * cells * sizeof(uint32_t);
+        const size_t sk = cms_local_lds(h.kbl, cells);
         const uint32_t nsmall = hs.nlocal - hs.nlocal_big;
         const uint32_t *lsmall = hs.local.p, *lbig = hs.local.p + nsmall;
         if (hs.cms) {
@@ -377,9 +397,9 @@ void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
             } else {
                 // resident workgroups loop over the vertices, each with its stash
                 h.stash = hs.stash.p;
-                h.stash_stride = (uint64_t)CMS_LOCAL * HEAVY_KB_CMS;
+                h.stash_stride = (uint64_t)CMS_LOCAL * HEAVY_KB_CMS_LOCAL;
                 const uint32_t g = hs.stash_blocks ? std::min(hs.nlocal, hs.stash_blocks) : hs.nlocal;
-                hipLaunchKernelGGL((k_heavy_local<1, HEAVY_KB_CMS, 4>), dim3(g), b, sk, stream, h, hs.local.p,
+                hipLaunchKernelGGL((k_heavy_local<1, HEAVY_KB_CMS_LOCAL, 4>), dim3(g), b, sk, stream, h, hs.local.p,
                                    hs.nlocal);
             }
         } else {
@@ -387,12 +407,12 @@ void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
             const size_t t1 = 2 * CHUNK * sizeof(uint64_t), t2 = 4 * CHUNK * sizeof(uint64_t);
             if (nsmall) {
                 if (one) hipLaunchKernelGGL((k_heavy_local<0, 1, 4>), dim3(nsmall), b, t1, stream, h, lsmall, nsmall);
-                else hipLaunchKernelGGL((k_heavy_local<0, HEAVY_KB_MAX, 4>), dim3(nsmall), b, t1, stream, h, lsmall, nsmall);
+                else hipLaunchKernelGGL((k_heavy_local<0, HEAVY_KB_LOCAL, 4>), dim3(nsmall), b, t1, stream, h, lsmall, nsmall);
             }
             if (hs.nlocal_big) {
                 const dim3 g(hs.nlocal_big);
                 if (one) hipLaunchKernelGGL((k_heavy_local<0, 1, 8>), g, b, t2, stream, h, lbig, hs.nlocal_big);
-                else hipLaunchKernelGGL((k_heavy_local<0, HEAVY_KB_CMS, 8>), g, b, t2, stream, h, lbig, hs.nlocal_big);
+                else hipLaunchKernelGGL((k_heavy_local<0, HEAVY_KB_LOCAL, 8>), g, b, t2, stream, h, lbig, hs.nlocal_big);
             }
         }
     }
@@ -567,9 +587,9 @@ int lpa_engine_create(int device, lpa_engine **out) {
             allow_group_lds<BLOCK>(dyn);
             const void *heavy_fns[] = {(const void *)k_heavy_chunks<0, 1>, (const void *)k_heavy_chunks<0, HEAVY_KB_MAX>,
                                        (const void *)k_heavy_chunks<1, 1>, (const void *)k_heavy_chunks<1, HEAVY_KB_CMS>,
-                                       (const void *)k_heavy_local<0, 1, 4>, (const void *)k_heavy_local<0, HEAVY_KB_MAX, 4>,
-                                       (const void *)k_heavy_local<0, 1, 8>, (const void *)k_heavy_local<0, HEAVY_KB_CMS, 8>,
-                                       (const void *)k_heavy_local<1, 1, 4>, (const void *)k_heavy_local<1, HEAVY_KB_CMS, 4>};
+                                       (const void *)k_heavy_local<0, 1, 4>, (const void *)k_heavy_local<0, HEAVY_KB_LOCAL, 4>,
+                                       (const void *)k_heavy_local<0, 1, 8>, (const void *)k_heavy_local<0, HEAVY_KB_LOCAL, 8>,
+                                       (const void *)k_heavy_local<1, 1, 4>, (const void *)k_heavy_local<1, HEAVY_KB_CMS_LOCAL, 4>};
             for (const void *f : heavy_fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
             (void)hipGetLastError();
         } catch (...) {
@@ -822,6 +842,7 @@ static void build_heavy(lpa_engine *e) {
             std::vector<uint32_t> pos, local, local_big, chunked, slot;
             std::vector<uint64_t> loff(1, 0), toff(1, 0), cb;
             std::vector<uint32_t> ch;
+            uint64_t max_chunked = 0;
             // one workgroup holds the whole count: exact up to 2*CHUNK ids
             // (LDS table of up to 4*CHUNK slots, when the LDS holds it), CMS
             // up to CMS_LOCAL ids (LDS sketches)
@@ -837,7 +858,10 @@ static void build_heavy(lpa_engine *e) {
                     pos.push_back(q);
                     const bool is_local = d <= local_max;
                     slot.push_back(is_local ? 0u : (uint32_t)chunked.size());
-                    if (!is_local) chunked.push_back(i);
+                    if (!is_local) {
+                        chunked.push_back(i);
+                        max_chunked = std::max(max_chunked, d);
+                    }
                     else if (!cms && d > CHUNK) local_big.push_back(i);
                     else local.push_back(i);
                     if (!is_local)
@@ -924,18 +948,22 @@ static void build_heavy(lpa_engine *e) {
                 hs->kb = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(e->K, table_budget / std::max<uint64_t>(per_vec, 1)));
                 hs->toff.upload(toff.data(), toff.size(), e->stream);
                 hs->tab.alloc(std::max<uint64_t>(toff.back(), 1) * hs->kb);
+                // counts share the slot's count word with the generation
+                const char *tags = std::getenv("LPA_TABLE_TAGS");
+                hs->tagged = max_chunked <= COUNT_MASK && !(tags && std::strcmp(tags, "0") == 0);
+                hs->epoch.alloc(1);
+                HIPCHK(hipMemsetAsync(hs->epoch.p, 0, sizeof(uint32_t), e->stream));
             }
             hs->result.alloc((uint64_t)std::max<uint32_t>(hs->nchunked, 1) * hs->kb);
             if (hs->cms && !hs->csr_part && hs->nlocal && e->K > 1) {
-                // one stash of CMS_LOCAL rows of HEAVY_KB_CMS labels per workgroup the CUs hold at once
+                // one stash of CMS_LOCAL rows of HEAVY_KB_CMS_LOCAL labels per workgroup the CUs hold at once
                 const size_t cells = (size_t)e->cms_depth * e->cms_width;
-                const size_t kbs = std::max<size_t>(1, std::min<size_t>({(size_t)e->K, (size_t)HEAVY_KB_CMS,
-                                                                          (size_t)lds_budget(e) / (cells * 4)}));
+                const uint32_t kbl = cms_local_kb(e->K, cells, lds_budget(e));
                 int per_cu = 0;
                 HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    &per_cu, (const void *)k_heavy_local<1, HEAVY_KB_CMS, 4>, HBLOCK, kbs * cells * 4));
+                    &per_cu, (const void *)k_heavy_local<1, HEAVY_KB_CMS_LOCAL, 4>, HBLOCK, cms_local_lds(kbl, cells)));
                 hs->stash_blocks = (uint32_t)std::min<uint64_t>(hs->nlocal, (uint64_t)std::max(per_cu, 1) * e->num_cu);
-                hs->stash.alloc((uint64_t)hs->stash_blocks * CMS_LOCAL * HEAVY_KB_CMS);
+                hs->stash.alloc((uint64_t)hs->stash_blocks * CMS_LOCAL * HEAVY_KB_CMS_LOCAL);
             }
             e->heavy.push_back(std::move(hs));
         }

@@ -32,8 +32,10 @@ constexpr uint32_t CHUNK = 4096;
 // workgroup size of the chunk and one-workgroup heavy kernels (large LDS
 // tables and sketches; many waves share them)
 constexpr int HBLOCK = 1024;
-// CMS vertices up to this degree are counted by one workgroup (LDS sketch).
-constexpr uint32_t CMS_LOCAL = 65536;
+// CMS vertices up to this degree are counted by one workgroup (LDS
+// sketches; with several label vectors their cells are 16-bit, two vectors
+// to a word, and no cell of a list this short can reach 2^16).
+constexpr uint32_t CMS_LOCAL = 65535;
 
 struct HeavyArgs {
     StepArgs s;                                // words/offs/vid, row/nbr/cvid, labels, CMS hashes
@@ -53,10 +55,54 @@ struct HeavyArgs {
     int cms;                                   // 1: CMS mode, 0: exact
     uint32_t k0, kb;                           // label vectors of this batch
     uint32_t kbs;                              // CMS: label vectors per sketch pass (sketches side by side in LDS)
+    uint32_t kbl;                              // CMS one-workgroup kernel: vectors per pass (packed 16-bit cells)
     const uint32_t *__restrict__ slot;         // chunked vertex i -> its sketch / result slot
     uint32_t *stash;                           // CMS, several vectors: per-workgroup label stash (or NULL)
     uint64_t stash_stride;                     // words per workgroup
+    uint32_t *epoch;                           // exact mode, tagged tables: passes completed (device word)
+    int tagged;                                // exact mode: HBM table slots carry the pass's generation
+    uint32_t kb_alloc;                         // exact mode: vectors the table buffer is laid out for
 };
+
+// Exact-mode HBM tables with generation tags.  A slot's count word holds
+// (generation << 24 | count) and a slot whose generation is not the current
+// pass's counts as empty, so a pass needs no clearing of its tables: only
+// when the 8-bit generation wraps (every TABLE_GENS passes) are the tables
+// cleared in full.  Used when every chunked list is shorter than 2^24.
+// The generation is derived on the device from the pass counter, which the
+// pass's last kernel advances, so a captured iteration replays correctly.
+constexpr uint32_t TABLE_GENS = 255;
+constexpr uint32_t COUNT_BITS = 24;
+constexpr uint32_t COUNT_MASK = (1u << COUNT_BITS) - 1;
+
+__device__ __forceinline__ uint32_t table_gen(const HeavyArgs &h) { return *h.epoch % TABLE_GENS + 1; }
+
+// tab_add for a tagged HBM table: claims a stale or empty slot with one
+// 64-bit CAS, else adds to the matching label's count word.  Returns the
+// label's count after the addition.
+__device__ __forceinline__ uint32_t tab_add_tagged(uint64_t *tab, uint32_t logT, uint32_t lab, uint32_t cnt,
+                                                   uint32_t gen) {
+    const uint64_t mask = (1ull << logT) - 1;
+    uint64_t hh = logT ? (lab * 2654435769u) >> (32 - logT) : 0;
+    uint64_t cur = __hip_atomic_load(&tab[hh], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (;;) {
+        const uint32_t slab = (uint32_t)(cur >> 32), cw = (uint32_t)cur;
+        if (slab != UNL && (cw >> COUNT_BITS) == gen) {
+            if (slab == lab) {
+                uint32_t *w = reinterpret_cast<uint32_t *>(&tab[hh]);   // little endian: w[0] count word
+                return ((atomicAdd(w, cnt) + cnt) & COUNT_MASK);
+            }
+            hh = (hh + 1) & mask;
+            cur = __hip_atomic_load(&tab[hh], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            continue;
+        }
+        const uint64_t want = ((uint64_t)lab << 32) | ((uint64_t)gen << COUNT_BITS) | cnt;
+        const uint64_t old = atomicCAS((unsigned long long *)&tab[hh], (unsigned long long)cur,
+                                       (unsigned long long)want);
+        if (old == cur) return cnt;
+        cur = old;   // taken meanwhile: look at what is there now
+    }
+}
 
 __device__ __forceinline__ const uint32_t *heavy_ids(const HeavyArgs &h, uint32_t i) {
     return h.csr ? h.s.nbr + h.s.row[h.pos[i]] : h.list + h.loff[i];
@@ -176,8 +222,13 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_clear(HeavyArgs h, const uint32
         const uint64_t cells = ((uint64_t)h.s.cms_depth << h.s.cms_logw) * h.kb;
         uint32_t *sk = h.sketch + (uint64_t)sl * cells;
         for (uint64_t c = threadIdx.x; c < cells; c += BLOCK) sk[c] = 0;
-    } else {
+    } else if (!h.tagged) {
         for (uint64_t c = h.toff[i] * h.kb + threadIdx.x; c < h.toff[i + 1] * h.kb; c += BLOCK) h.tab[c] = EMPTY_SLOT;
+    } else if (*h.epoch % TABLE_GENS == 0) {
+        // generation 1 again: clear the vertex's share of the whole buffer
+        // (laid out for kb_alloc vectors, whatever this pass's kb)
+        for (uint64_t c = h.toff[i] * h.kb_alloc + threadIdx.x; c < h.toff[i + 1] * h.kb_alloc; c += BLOCK)
+            h.tab[c] = EMPTY_SLOT;
     }
 }
 
@@ -199,13 +250,20 @@ constexpr int HEAVY_IPL = CHUNK / HBLOCK;
 // serve up to KB label vectors (kbs sketches side by side) per round.
 constexpr int CMS_R = 4;
 // Label vectors per staged batch of the heavy kernels (template KB): 1 for
-// a single vector, else 8 for exact counting and 4 for sketches (four
-// sketches side by side leave LDS for two workgroups per CU).
+// a single vector, else 8 for the exact chunk kernel, 4 for sketches (four
+// sketches side by side leave LDS for two workgroups per CU) and 4 for the
+// exact one-workgroup kernels (16 staged labels per lane fit the 64
+// registers of eight waves per SIMD without spilling).
 constexpr int HEAVY_KB_MAX = 8;
 constexpr int HEAVY_KB_CMS = 4;
+constexpr int HEAVY_KB_LOCAL = 4;
+// The one-workgroup CMS kernel packs two vectors' 16-bit cells in a word,
+// so eight sketches take the LDS of four and a row is gathered once per 8.
+constexpr int HEAVY_KB_CMS_LOCAL = 8;
 
 // One sketch pass over ids[0..n) for vectors k0 .. k0+kb-1 (kb <= KB;
-// sketch s at sk + s*cells).  ADD: add every label to its sketch (LDS,
+// sketch s at sk + s*cells, or with PACK the 16-bit half s&1 of the words
+// at sk + (s>>1)*cells).  ADD: add every label to its sketch (LDS,
 // pre-aggregated per wave); else: estimate every label from its sketch and
 // keep the best (count << 32 | ~label) per vector in best[s].
 // stash (HBM, n*KB words, or NULL): the ADD pass stores every gathered
@@ -213,10 +271,12 @@ constexpr int HEAVY_KB_CMS = 4;
 // order instead of gathering the rows again (a second random gather of a
 // graph whose labels do not fit the caches costs a full memory access per
 // arc).  With one label vector the decoded list itself is the stash.
-template <bool ADD, int KB>
+template <bool ADD, int KB, bool PACK = false>
 __device__ __forceinline__ void cms_rounds(const StepArgs &a, const uint32_t *__restrict__ ids, uint32_t n,
                                            uint32_t k0, uint32_t kb, uint32_t *sk, uint32_t cells,
                                            uint64_t (&best)[KB], uint32_t *stash = nullptr) {
+    // neighbours per lane and round: fewer when a row carries 8 labels
+    constexpr int R = KB > 4 ? 1 : CMS_R;
     // the row hashes stay in scalar registers for the whole pass (a runtime
     // row index into the argument block would reload them per label)
     constexpr int MAXD = 8;
@@ -230,14 +290,20 @@ __device__ __forceinline__ void cms_rounds(const StepArgs &a, const uint32_t *__
     auto col = [&](int r, uint32_t lab) -> uint32_t {
         return logw ? (uint32_t)((ha[r] * (uint64_t)lab + hb[r]) >> sh) : 0u;
     };
-    for (uint32_t i0 = 0; i0 < n; i0 += CMS_R * HBLOCK) {
-        uint32_t labs[CMS_R][KB];
+    for (uint32_t i0 = 0; i0 < n; i0 += R * HBLOCK) {
+        uint32_t labs[R][KB];
 #pragma unroll
-        for (int t = 0; t < CMS_R; t++) {
+        for (int t = 0; t < R; t++) {
             const uint32_t i = i0 + t * HBLOCK + threadIdx.x;
             if (i < n) {
                 if (stash && !ADD) {
-                    if (KB == 4) {
+                    if (KB == 8) {
+                        const uint4 w0 = reinterpret_cast<const uint4 *>(stash)[2 * i];
+                        const uint4 w1 = reinterpret_cast<const uint4 *>(stash)[2 * i + 1];
+                        const uint32_t v8[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+#pragma unroll
+                        for (int q = 0; q < KB; q++) labs[t][q] = v8[q & 7];
+                    } else if (KB == 4) {
                         const uint4 w = reinterpret_cast<const uint4 *>(stash)[i];
                         labs[t][0] = w.x;
                         labs[t][KB > 1 ? 1 : 0] = w.y;
@@ -250,7 +316,11 @@ __device__ __forceinline__ void cms_rounds(const StepArgs &a, const uint32_t *__
                 } else {
                     load_row<KB>(a, ids[i], k0, kb, labs[t]);
                     if (stash) {
-                        if (KB == 4) {
+                        if (KB == 8) {
+                            uint4 *st = reinterpret_cast<uint4 *>(stash) + 2 * i;
+                            st[0] = make_uint4(labs[t][0], labs[t][1 % KB], labs[t][2 % KB], labs[t][3 % KB]);
+                            st[1] = make_uint4(labs[t][4 % KB], labs[t][5 % KB], labs[t][6 % KB], labs[t][7 % KB]);
+                        } else if (KB == 4) {
                             reinterpret_cast<uint4 *>(stash)[i] =
                                 make_uint4(labs[t][0], labs[t][KB > 1 ? 1 : 0], labs[t][KB > 2 ? 2 : 0],
                                            labs[t][KB > 3 ? 3 : 0]);
@@ -268,9 +338,10 @@ __device__ __forceinline__ void cms_rounds(const StepArgs &a, const uint32_t *__
 #pragma unroll
         for (int q = 0; q < KB; q++) {
             if ((uint32_t)q >= kb) continue;
-            uint32_t *skq = sk + (uint64_t)q * cells;
+            uint32_t *skq = sk + (uint64_t)(PACK ? q >> 1 : q) * cells;
+            const uint32_t half = PACK ? 16u * (q & 1) : 0u;   // a constant once unrolled
 #pragma unroll
-            for (int t = 0; t < CMS_R; t++) {
+            for (int t = 0; t < R; t++) {
                 const uint32_t lab = labs[t][q];
                 if (lab == UNL) continue;
                 if (ADD) {
@@ -279,14 +350,15 @@ __device__ __forceinline__ void cms_rounds(const StepArgs &a, const uint32_t *__
 #pragma unroll
                     for (int r = 0; r < MAXD; r++) {
                         if ((uint32_t)r >= depth) break;
-                        atomicAdd(&skq[((uint32_t)r << logw) + col(r, lab)], cnt);
+                        atomicAdd(&skq[((uint32_t)r << logw) + col(r, lab)], cnt << half);
                     }
                 } else {
                     uint32_t est = UNL;
 #pragma unroll
                     for (int r = 0; r < MAXD; r++) {
                         if ((uint32_t)r >= depth) break;
-                        const uint32_t x = skq[((uint32_t)r << logw) + col(r, lab)];
+                        const uint32_t x = PACK ? (skq[((uint32_t)r << logw) + col(r, lab)] >> half) & 0xFFFFu
+                                                : skq[((uint32_t)r << logw) + col(r, lab)];
                         est = x < est ? x : est;
                     }
                     const uint64_t key = ((uint64_t)est << 32) | (uint32_t)~lab;
@@ -319,6 +391,7 @@ __global__ void __launch_bounds__(HBLOCK) k_heavy_chunks(HeavyArgs h) {
     const uint64_t t0 = a.rec ? cycles() : 0;
     const uint32_t sl = h.slot[i];
     if (PASS == 0) {
+        const uint32_t gen = h.tagged ? table_gen(h) : 0;
         const uint64_t T = h.toff[i + 1] - h.toff[i];
         const uint32_t logT = 63 - __builtin_clzll(T);
         constexpr uint32_t LT = 2 * CHUNK, LOG_LT = 31 - __builtin_clz(LT);
@@ -345,7 +418,8 @@ __global__ void __launch_bounds__(HBLOCK) k_heavy_chunks(HeavyArgs h) {
                         const uint64_t e = ltab[x];
                         if ((uint32_t)(e >> 32) == UNL) continue;
                         const uint32_t lab = (uint32_t)(e >> 32);
-                        const uint32_t cc = tab_add(tab, logT, lab, (uint32_t)e);
+                        const uint32_t cc = h.tagged ? tab_add_tagged(tab, logT, lab, (uint32_t)e, gen)
+                                                     : tab_add(tab, logT, lab, (uint32_t)e);
                         const uint64_t key = ((uint64_t)cc << 32) | (uint32_t)~lab;
                         best = key > best ? key : best;
                     }
@@ -390,7 +464,9 @@ __global__ void __launch_bounds__(HBLOCK) k_heavy_chunks(HeavyArgs h) {
 }
 
 template <int CMS, int KB, int IPL>
-__device__ __forceinline__ void heavy_local_vertex(const HeavyArgs &h, uint32_t i, uint64_t *lds, uint64_t *red) {
+__device__ __forceinline__ void heavy_local_vertex(const HeavyArgs &h, uint32_t i, uint64_t *lds, uint64_t *red,
+                                                   uint32_t *olds) {
+    static_assert(KB <= HEAVY_KB_MAX, "olds[] holds one batch");
     const uint32_t d = heavy_deg(h, i);
     const uint32_t *ids = heavy_ids(h, i);
     const StepArgs &a = h.s;
@@ -398,19 +474,29 @@ __device__ __forceinline__ void heavy_local_vertex(const HeavyArgs &h, uint32_t 
     const uint32_t p = h.pos[i];
     const uint32_t v = h.csr ? a.cvid[p] : a.vid[p];
     uint32_t nch = 0;
-    auto finish = [&](uint32_t k, uint64_t best) {   // thread 0
-        const uint64_t idx = (uint64_t)v * a.K + k;
-        const uint32_t old = a.lin[idx];
+    // a batch's current labels are read into olds[] (LDS) by its first kb
+    // lanes before the counting, so picking a vector's label costs thread 0
+    // no memory round trip
+    auto prefetch = [&](uint32_t k0, uint32_t kb) {
+        if (threadIdx.x < kb) olds[threadIdx.x] = a.lin[(uint64_t)v * a.K + k0 + threadIdx.x];
+    };
+    auto finish = [&](uint32_t k0, uint32_t q, uint64_t best) {   // thread 0
+        const uint64_t idx = (uint64_t)v * a.K + k0 + q;
+        const uint32_t old = olds[q];
         const uint32_t nl = pick(best, old);
         a.lout[idx] = nl;
         nch += nl != old;
     };
     if (CMS) {
+        // several vectors: 16-bit cells, two vectors to a word
+        constexpr bool PACK = KB > 1;
         const uint32_t cells = a.cms_depth << a.cms_logw;
         uint32_t *sk = (uint32_t *)lds;
-        for (uint32_t k0 = 0; k0 < a.K; k0 += h.kbs) {
-            const uint32_t kb = a.K - k0 < h.kbs ? a.K - k0 : h.kbs;
-            for (uint32_t x = threadIdx.x; x < kb * cells; x += HBLOCK) sk[x] = 0;
+        for (uint32_t k0 = 0; k0 < a.K; k0 += h.kbl) {
+            const uint32_t kb = a.K - k0 < h.kbl ? a.K - k0 : h.kbl;
+            prefetch(k0, kb);
+            const uint32_t words = (PACK ? (kb + 1) / 2 : kb) * cells;
+            for (uint32_t x = threadIdx.x; x < words; x += HBLOCK) sk[x] = 0;
             __syncthreads();
             uint64_t best[KB] = {};
             // stash: the decoded list itself for one vector, else this
@@ -418,14 +504,14 @@ __device__ __forceinline__ void heavy_local_vertex(const HeavyArgs &h, uint32_t 
             uint32_t *stash = h.csr ? nullptr
                             : KB == 1 && a.K == 1 ? const_cast<uint32_t *>(ids)
                             : h.stash ? h.stash + (uint64_t)blockIdx.x * h.stash_stride : nullptr;
-            cms_rounds<true, KB>(a, ids, d, k0, kb, sk, cells, best, stash);
+            cms_rounds<true, KB, PACK>(a, ids, d, k0, kb, sk, cells, best, stash);
             __syncthreads();
-            cms_rounds<false, KB>(a, ids, d, k0, kb, sk, cells, best, stash);
+            cms_rounds<false, KB, PACK>(a, ids, d, k0, kb, sk, cells, best, stash);
 #pragma unroll
             for (int q = 0; q < KB; q++) {
                 if ((uint32_t)q >= kb) continue;
                 const uint64_t m = block_max(best[q], red);
-                if (threadIdx.x == 0) finish(k0 + q, m);
+                if (threadIdx.x == 0) finish(k0, q, m);
             }
         }
     } else {
@@ -435,6 +521,7 @@ __device__ __forceinline__ void heavy_local_vertex(const HeavyArgs &h, uint32_t 
         for (uint32_t k0 = 0; k0 < a.K; k0 += KB) {
             const uint32_t kb = a.K - k0 < (uint32_t)KB ? a.K - k0 : (uint32_t)KB;
             uint64_t best = 0;
+            prefetch(k0, kb);
             count_staged<HBLOCK, IPL, KB>(
                 a, ids, d, threadIdx.x, k0, kb,
                 [&](int) {
@@ -456,7 +543,7 @@ __device__ __forceinline__ void heavy_local_vertex(const HeavyArgs &h, uint32_t 
                 },
                 [&](int s) {
                     const uint64_t m = block_max(best, red);
-                    if (threadIdx.x == 0) finish(k0 + s, m);
+                    if (threadIdx.x == 0) finish(k0, s, m);
                 });
         }
     }
@@ -470,20 +557,23 @@ __device__ __forceinline__ void heavy_local_vertex(const HeavyArgs &h, uint32_t 
 // counts its whole list -- both CMS passes, or the exact table -- and writes
 // the labels itself, with no HBM table, sketch or merge.  local[] lists the
 // heavy-vertex indices; IPL = staged ids per lane (exact).
-// (Eight waves per SIMD requested: two 1024-lane workgroups per CU when their
-// tables fit, at the price of a few spilled registers in the widest case.)
+// (Eight waves per SIMD requested -- two 1024-lane workgroups per CU -- when
+// the tables leave LDS for two: IPL = 4; the 128 KB tables of IPL = 8 admit
+// one workgroup per CU, so those instances take four waves per SIMD and the
+// registers that frees.)
 template <int CMS, int KB, int IPL>
-__global__ void __launch_bounds__(HBLOCK, 8) k_heavy_local(HeavyArgs h, const uint32_t *__restrict__ local,
+__global__ void __launch_bounds__(HBLOCK, IPL == 8 ? 4 : 8) k_heavy_local(HeavyArgs h, const uint32_t *__restrict__ local,
                                                        uint32_t nlocal) {
     extern __shared__ uint64_t lds[];
     __shared__ uint64_t red[HBLOCK / 64];
+    __shared__ uint32_t olds[HEAVY_KB_MAX];
     if (CMS) {
         // workgroups loop over the vertices (the grid may be smaller than
         // the list when each workgroup owns a stash region)
         for (uint32_t j = blockIdx.x; j < nlocal; j += gridDim.x)
-            heavy_local_vertex<CMS, KB, IPL>(h, local[j], lds, red);
+            heavy_local_vertex<CMS, KB, IPL>(h, local[j], lds, red, olds);
     } else if (blockIdx.x < nlocal) {
-        heavy_local_vertex<CMS, KB, IPL>(h, local[blockIdx.x], lds, red);
+        heavy_local_vertex<CMS, KB, IPL>(h, local[blockIdx.x], lds, red, olds);
     }
 }
 
@@ -504,6 +594,8 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_finish(HeavyArgs h, const uint3
         ch = nl != old;
     }
     count_changes(ch, h.s.changed);
+    // the pass is complete: the next one counts under the next generation
+    if (!CMS && h.tagged && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(h.epoch, 1u);
 }
 
 }  // namespace lpa

@@ -476,7 +476,7 @@ __global__ void __launch_bounds__(BLOCK) k_group(StepArgs a, uint32_t logT, uint
         }
         for (uint32_t k0 = 0; k0 < a.K; k0 += KB) {
             const uint32_t kb = a.K - k0 < (uint32_t)KB ? a.K - k0 : (uint32_t)KB;
-            uint64_t best = 0;
+            uint64_t best = 0, mine = 0;   // lane gl < kb keeps vector k0+gl's result
             count_staged<G, IPL, KB>(
                 a, ids, d, gl, k0, kb,
                 [&](int) {
@@ -504,15 +504,18 @@ __global__ void __launch_bounds__(BLOCK) k_group(StepArgs a, uint32_t logT, uint
 #pragma unroll
                         for (int w = 0; w < BLOCK / 64; w++) best = red[w] > best ? red[w] : best;
                     }
-                    if (gl == 0 && live) {
-                        const uint64_t idx = (uint64_t)v * a.K + k0 + s;
-                        const uint32_t old = a.lin[idx];
-                        const uint32_t nl = pick(best, old);
-                        a.lout[idx] = nl;
-                        nch += nl != old;
-                    }
+                    if (gl == (uint32_t)s) mine = best;
                     group_sync<G>();   // table / red reused next
                 });
+            // the batch's labels are picked and written by kb lanes at once
+            // (one memory round trip per batch, not per vector)
+            if (gl < kb && live) {
+                const uint64_t idx = (uint64_t)v * a.K + k0 + gl;
+                const uint32_t old = a.lin[idx];
+                const uint32_t nl = pick(mine, old);
+                a.lout[idx] = nl;
+                nch += nl != old;
+            }
         }
         if (a.rec && live && gl == 0) a.rec[v] = (cycles() - t0) * G;
     }

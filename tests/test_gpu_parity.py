@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 import lpa_amd as L
+import pyoracle as O
 from gpu_common import make_engine, oracle_view, run_and_compare
 from helpers import golden_cases, load_golden
 
@@ -155,3 +156,37 @@ def test_heavy_local_and_chunked_paths():
     run_and_compare(cgr, 3, K=3, cms_thresh=2500, cms_depth=3, cms_width=1024, cms_seed=6)
     run_and_compare(cgr, 3, K=2)
     run_and_compare(g.encode(16, 128), 3, cms_thresh=1000, cms_depth=4, cms_width=512, cms_seed=8)
+
+
+def test_exact_tables_across_generation_wrap():
+    """exact-mode HBM tables carry an 8-bit pass generation and are cleared
+    only when it wraps (every 255 passes): after 252 steps, eight steps from
+    fresh random labels (few distinct values, so stale counts would change
+    modes) straddle the wrap, each against the oracle"""
+    g = _hub_graph()
+    cgr = g.encode(0, 128)
+    e = make_engine(cgr)
+    e.labels_init()
+    for _ in range(252):
+        e.nova_iterate()
+    view = oracle_view(cgr)
+    c = O.cfg(K=1)
+    rng = np.random.default_rng(5)
+    for it in range(8):
+        lab = rng.integers(0, 40, cgr.n).astype(np.uint32)
+        e.labels_init(lab)
+        ch = e.nova_iterate()
+        exp, och = O.step_cgr(view, lab, c, None)
+        assert np.array_equal(e.labels(), exp), it
+        assert ch == och, it
+    e.close()
+
+
+def test_exact_tables_untagged(monkeypatch):
+    """LPA_TABLE_TAGS=0 selects the per-pass clearing of the exact HBM tables
+    (the layout used when a chunked list is 2^24 ids or longer)"""
+    monkeypatch.setenv("LPA_TABLE_TAGS", "0")
+    g = _hub_graph()
+    cgr = g.encode(0, 128)
+    run_and_compare(cgr, 3)
+    run_and_compare(cgr, 2, K=3)
