@@ -15,6 +15,8 @@ engine's stream.  Edges/s = K * m / t_step.  Reported:
   degree_k1        degree-binned fused step, K = 1 (the plain kernel)
   degree_k8        degree-binned fused step, K = 8 (isolates the schedule's gain)
   cglp_k1          the two-kernel method, K = 1
+  csr_k1           comparator: LP over the uncompressed CSR (decoded once,
+                   no decode in the step), K = 1
 The roofline object describes the dominant kernel family of the headline run;
 cpu_baseline times the CPU oracle's decode+LP step on the host cores over a
 bounded random sample of the same graph's vertices.
@@ -104,7 +106,7 @@ def alg_bytes(st, K):
 def timed(e, steps, warmup, method="nova", labels=None):
     e.labels_init(labels)
     for _ in range(warmup):
-        e.nova_iterate() if method == "nova" else e.cglp_iterate()
+        {"nova": e.nova_iterate, "cglp": e.cglp_iterate, "csr": e.csr_iterate}[method]()
     ms, changed = e.time_iterations(steps, method)
     return ms, changed, e.kernel_times()
 
@@ -178,6 +180,11 @@ def main():
             ms2, _, _ = timed(e, max(2, args.steps // 2), 1, "cglp", labels=lab1)
             res["cglp_k1"] = dict(edges_per_s=m / (float(np.mean(ms2)) / 1e3), ms_per_step=float(np.mean(ms2)))
             log(f"cglp K=1: {np.mean(ms2):.3f} ms/step")
+            # comparator: LP over the uncompressed CSR (no decode in the step)
+            ms3, _, _ = timed(e, max(2, args.steps // 2), 1, "csr", labels=lab1)
+            res["csr_k1"] = dict(edges_per_s=m / (float(np.mean(ms3)) / 1e3), ms_per_step=float(np.mean(ms3)),
+                                 note="uncompressed CSR, same kernels; the CSR adds 4 B per arc of HBM")
+            log(f"uncompressed CSR K=1: {np.mean(ms3):.3f} ms/step")
     head_ms = None
     if not args.headline_only or args.no_learned:
         # degree-binned, K vectors

@@ -30,6 +30,9 @@
  *   lpa_cglp_iterate            CGLPPropagator::iterate: decode, then the
  *                               gather/sort/scan/compute_max_labels LP stage
  *                               (baselines/cglp/cglp_propagator.cuh:205-335)
+ *   lpa_csr_iterate             the uncompressed-CSR comparator of baselines/g-sort
+ *                               (InCoreLIDPP::iterate, incore_li.cuh:132-186: LP
+ *                               over the plain CSR), here with the hash kernels
  *   lpa_cglp_csr_get            (no counterpart) copy of the decoded CSR, for checks
  *   lpa_time_iterations         the per-iteration Timer around iterate
  *                               (nova_propogator.cuh:114-125), with HIP events
@@ -159,6 +162,10 @@ int lpa_nova_iterate(lpa_engine *e, uint64_t *changed);
 int lpa_cglp_decode(lpa_engine *e);
 /* One synchronous step the two-kernel way: decode to CSR, then LP over it. */
 int lpa_cglp_iterate(lpa_engine *e, uint64_t *changed);
+/* One synchronous step of LP over the uncompressed CSR (the compressed part
+ * decoded once, then reused; no per-step decode): the comparator that shows
+ * what reading the compressed stream costs.  Same labels as the other two. */
+int lpa_csr_iterate(lpa_engine *e, uint64_t *changed);
 /* Copy of the device CSR by vertex id: row (n+1), col (m). */
 int lpa_cglp_csr_get(lpa_engine *e, uint64_t *row, uint32_t *col, uint64_t *m);
 
@@ -206,7 +213,7 @@ int lpa_mlp_train(lpa_mlp *m, const float *x, const float *target, const uint8_t
  * part keep their degree family. */
 int lpa_schedule_learned(lpa_engine *e, lpa_mlp *m);
 
-/* Runs iters steps (method 0 = NOVA fused, 1 = CGLP) and reports the
+/* Runs iters steps (method 0 = NOVA fused, 1 = CGLP, 2 = uncompressed CSR) and reports the
  * HIP-event time of each (ms) and the changed counts; captures the step in
  * a HIP graph when use_graph != 0. */
 int lpa_time_iterations(lpa_engine *e, int method, int iters, int use_graph,

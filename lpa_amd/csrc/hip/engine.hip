@@ -194,7 +194,7 @@ struct lpa_engine {
 
     // captured steps: graphs[method][cur]
     bool graph_mode = false;
-    hipGraphExec_t graphs[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    hipGraphExec_t graphs[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
     void drop_graphs() {
         for (auto &m : graphs)
             for (auto &g : m)
@@ -412,7 +412,7 @@ void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
             if (hs.nlocal_big) {
                 const dim3 g(hs.nlocal_big);
                 if (one) hipLaunchKernelGGL((k_heavy_local<0, 1, 8>), g, b, t2, stream, h, lbig, hs.nlocal_big);
-                else hipLaunchKernelGGL((k_heavy_local<0, HEAVY_KB_LOCAL, 8>), g, b, t2, stream, h, lbig, hs.nlocal_big);
+                else hipLaunchKernelGGL((k_heavy_local<0, HEAVY_KB_MAX, 8>), g, b, t2, stream, h, lbig, hs.nlocal_big);
             }
         }
     }
@@ -588,7 +588,7 @@ int lpa_engine_create(int device, lpa_engine **out) {
             const void *heavy_fns[] = {(const void *)k_heavy_chunks<0, 1>, (const void *)k_heavy_chunks<0, HEAVY_KB_MAX>,
                                        (const void *)k_heavy_chunks<1, 1>, (const void *)k_heavy_chunks<1, HEAVY_KB_CMS>,
                                        (const void *)k_heavy_local<0, 1, 4>, (const void *)k_heavy_local<0, HEAVY_KB_LOCAL, 4>,
-                                       (const void *)k_heavy_local<0, 1, 8>, (const void *)k_heavy_local<0, HEAVY_KB_LOCAL, 8>,
+                                       (const void *)k_heavy_local<0, 1, 8>, (const void *)k_heavy_local<0, HEAVY_KB_MAX, 8>,
                                        (const void *)k_heavy_local<1, 1, 4>, (const void *)k_heavy_local<1, HEAVY_KB_CMS_LOCAL, 4>};
             for (const void *f : heavy_fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
             (void)hipGetLastError();
@@ -1213,18 +1213,29 @@ static void require_ready(lpa_engine *e) {
 }
 
 static void cglp_step(lpa_engine *e);
+static void cglp_decode(lpa_engine *e);
+static void csr_lp(lpa_engine *e);
 
-// Enqueues one step (method 0 = fused, 1 = two-kernel) on the engine
-// stream: eagerly, or by replaying the step captured as a HIP graph.
+static void method_step(lpa_engine *e, int method) {
+    if (method == 0) e->step(false, nullptr);
+    else if (method == 1) cglp_step(e);
+    else csr_lp(e);
+}
+
+// Enqueues one step (method 0 = fused, 1 = two-kernel, 2 = LP over the
+// uncompressed CSR) on the engine stream: eagerly, or by replaying the step
+// captured as a HIP graph.
 static void enqueue_step(lpa_engine *e, int method) {
-    if (method == 1 && !e->have_csr) {
+    if (method >= 1 && !e->have_csr) {
         // the first decode allocates the CSR: never inside a capture
-        cglp_step(e);
-        return;
+        if (method == 1) {
+            cglp_step(e);
+            return;
+        }
+        cglp_decode(e);
     }
     if (!e->graph_mode) {
-        if (method == 0) e->step(false, nullptr);
-        else cglp_step(e);
+        method_step(e, method);
         return;
     }
     hipGraphExec_t &ge = e->graphs[method][e->cur];
@@ -1232,8 +1243,7 @@ static void enqueue_step(lpa_engine *e, int method) {
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
         try {
-            if (method == 0) e->step(false, nullptr);
-            else cglp_step(e);
+            method_step(e, method);
         } catch (...) {
             (void)hipStreamEndCapture(e->stream, &g);
             if (g) (void)hipGraphDestroy(g);
@@ -1318,6 +1328,13 @@ static void cglp_step(lpa_engine *e) {
                                    b.end);
         }
     }
+    csr_lp(e);
+}
+
+// The LP kernels over the CSR of every vertex (the decoded CSR of the
+// compressed part, which equals its uncompressed adjacency, and the small
+// CSR): CGLP's second kernel, and on its own the uncompressed comparator.
+static void csr_lp(lpa_engine *e) {
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, e->stream, e->counter.p, (uint64_t)1, 0ull);
     StepArgs a = e->base_args(false);
     StepArgs ac = e->base_args(true);
@@ -1335,6 +1352,14 @@ int lpa_cglp_iterate(lpa_engine *e, uint64_t *changed) {
     return guard([&] {
         require_ready(e);
         enqueue_step(e, 1);
+        finish_step(e, changed);
+    });
+}
+
+int lpa_csr_iterate(lpa_engine *e, uint64_t *changed) {
+    return guard([&] {
+        require_ready(e);
+        enqueue_step(e, 2);
         finish_step(e, changed);
     });
 }
@@ -1717,7 +1742,8 @@ int lpa_time_iterations(lpa_engine *e, int method, int iters, int use_graph, flo
         std::vector<double> kind_sum(LPA_NBIN_KINDS, 0.0);
         const bool had_graph_mode = e->graph_mode;
         try {
-            if (method == 1 && !e->have_csr) {
+            if (method < 0 || method > 2) throw Fail(LPA_ERR_ARG, "method: 0 NOVA, 1 CGLP, 2 CSR");
+            if (method >= 1 && !e->have_csr) {
                 cglp_decode(e);
                 e->sync();
             }
@@ -1726,7 +1752,7 @@ int lpa_time_iterations(lpa_engine *e, int method, int iters, int use_graph, flo
                 HIPCHK(hipEventRecord(ev[0], e->stream));
                 if (e->graph_mode) enqueue_step(e, method);
                 else if (method == 0) e->step(false, kev);
-                else cglp_step(e);
+                else method_step(e, method);
                 HIPCHK(hipEventRecord(ev[1], e->stream));
                 HIPCHK(hipMemcpyAsync(e->h_counter, e->counter.p, 8, hipMemcpyDeviceToHost, e->stream));
                 e->sync();

@@ -251,9 +251,10 @@ constexpr int HEAVY_IPL = CHUNK / HBLOCK;
 constexpr int CMS_R = 4;
 // Label vectors per staged batch of the heavy kernels (template KB): 1 for
 // a single vector, else 8 for the exact chunk kernel, 4 for sketches (four
-// sketches side by side leave LDS for two workgroups per CU) and 4 for the
-// exact one-workgroup kernels (16 staged labels per lane fit the 64
-// registers of eight waves per SIMD without spilling).
+// sketches side by side leave LDS for two workgroups per CU), 4 for the
+// exact one-workgroup kernel with 64 KB tables (16 staged labels per lane
+// fit the 64 registers of eight waves per SIMD without spilling) and 8 for
+// the one with 128 KB tables (one workgroup per CU: 128 registers).
 constexpr int HEAVY_KB_MAX = 8;
 constexpr int HEAVY_KB_CMS = 4;
 constexpr int HEAVY_KB_LOCAL = 4;

@@ -37,6 +37,7 @@ EXPORTS = {
     "lpa_nova_iterate": (C.c_int, [_VP, C.POINTER(C.c_uint64)]),
     "lpa_cglp_decode": (C.c_int, [_VP]),
     "lpa_cglp_iterate": (C.c_int, [_VP, C.POINTER(C.c_uint64)]),
+    "lpa_csr_iterate": (C.c_int, [_VP, C.POINTER(C.c_uint64)]),
     "lpa_cglp_csr_get": (C.c_int, [_VP, _VP, _VP, C.POINTER(C.c_uint64)]),
     "lpa_time_iterations": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, _VP, _VP]),
     "lpa_set_graph_mode": (C.c_int, [_VP, C.c_int]),
@@ -80,6 +81,10 @@ def load():
         f.restype, f.argtypes = res, args
     _LIB = L
     return L
+
+
+# lpa_time_iterations methods
+METHODS = {"nova": 0, "cglp": 1, "csr": 2}
 
 
 def _check(rc):
@@ -231,6 +236,12 @@ class Engine:
         _check(self._L.lpa_cglp_iterate(self._h, C.byref(c)))
         return c.value
 
+    def csr_iterate(self):
+        """one step of LP over the uncompressed CSR (decoded once, reused)"""
+        c = C.c_uint64(0)
+        _check(self._L.lpa_csr_iterate(self._h, C.byref(c)))
+        return c.value
+
     def cglp_csr(self):
         row = np.zeros(self.n + 1, np.uint64)
         m = C.c_uint64(0)
@@ -242,7 +253,7 @@ class Engine:
     def time_iterations(self, iters, method="nova", use_graph=False):
         ms = np.zeros(iters, np.float32)
         ch = np.zeros(iters, np.uint64)
-        _check(self._L.lpa_time_iterations(self._h, 0 if method == "nova" else 1, iters,
+        _check(self._L.lpa_time_iterations(self._h, METHODS[method], iters,
                                            int(use_graph), _p(ms), _p(ch)))
         return ms, ch
 

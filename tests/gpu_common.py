@@ -40,7 +40,7 @@ def run_and_compare(cgr, iters, labels0=None, method="nova", K=1, semi=False, cm
            else np.ascontiguousarray(labels0, np.uint32).reshape(-1).copy())
     seeds = (lab != O.UNLABELLED).astype(np.uint8) if semi else None
     for it in range(iters):
-        ch = e.nova_iterate() if method == "nova" else e.cglp_iterate()
+        ch = {"nova": e.nova_iterate, "cglp": e.cglp_iterate, "csr": e.csr_iterate}[method]()
         lab, och = O.step_cgr(view, lab, c, seeds)
         got = e.labels()
         if not np.array_equal(got, lab):

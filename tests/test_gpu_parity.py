@@ -51,6 +51,16 @@ def test_cglp_method_matches():
     run_and_compare(g.encode(0, 128), 4, method="cglp")
 
 
+def test_uncompressed_csr_method_matches():
+    """LP over the uncompressed CSR (decoded once, reused every step): the
+    same labels as NOVA and the oracle, exact and with CMS, K = 1 and 3"""
+    g = L.gen_rmat(13, 16, seed=13)
+    run_and_compare(g.encode(0, 128), 4, method="csr")
+    run_and_compare(g.encode(16, 128), 3, method="csr", K=3, cms_thresh=100, cms_depth=4, cms_width=256,
+                    cms_seed=3)
+    run_and_compare(_hub_graph().encode(0, 128), 2, method="csr")
+
+
 def test_cms_vertices():
     g = L.gen_rmat(14, 16, seed=14)
     run_and_compare(g.encode(0, 128), 4, cms_thresh=64, cms_depth=4, cms_width=256, cms_seed=7)
@@ -114,6 +124,12 @@ def test_graph_replay_matches_eager():
     graph.labels_init()
     for it in range(4):
         assert eager.cglp_iterate() == graph.cglp_iterate()
+        assert np.array_equal(eager.labels(), graph.labels()), it
+    # and the uncompressed-CSR comparator
+    eager.labels_init()
+    graph.labels_init()
+    for it in range(4):
+        assert eager.csr_iterate() == graph.csr_iterate()
         assert np.array_equal(eager.labels(), graph.labels()), it
 
 
