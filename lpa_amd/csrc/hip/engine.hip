@@ -1731,6 +1731,47 @@ int lpa_schedule_learned(lpa_engine *e, lpa_mlp *m) {
     });
 }
 
+int lpa_schedule_learned_judged(lpa_engine *e, lpa_mlp *m, int iters, float *learned_ms, float *degree_ms,
+                                int *kept_learned) {
+    return guard([&] {
+        require_ready(e);
+        if (!m || iters <= 0) throw Fail(LPA_ERR_ARG, "bad arguments");
+        // both layouts are timed from the same labels, which are put back
+        const uint64_t tot = e->n * e->K;
+        std::vector<uint32_t> saved(tot);
+        HIPCHK(hipMemcpy(saved.data(), e->L[e->cur].p, tot * 4, hipMemcpyDeviceToHost));
+        const int saved_cur = e->cur;
+        std::vector<float> ms(iters);
+        auto median_step = [&]() {
+            int rc = lpa_time_iterations(e, 0, iters, 0, ms.data(), nullptr);
+            if (rc) throw Fail(rc, g_err);
+            e->cur = saved_cur;
+            HIPCHK(hipMemcpy(e->L[e->cur].p, saved.data(), tot * 4, hipMemcpyHostToDevice));
+            std::vector<float> sorted(ms);
+            std::sort(sorted.begin(), sorted.end());
+            return sorted[sorted.size() / 2];
+        };
+        int rc = lpa_schedule_learned(e, m);
+        if (rc) throw Fail(rc, g_err);
+        const float tl = median_step();
+        const std::vector<uint8_t> learned_kind = e->kind_of;
+        const lpa_sched_params learned_p = e->sched_used;
+        lpa_sched_params q{32, 512, 0, 1, 128};
+        rc = lpa_schedule_degree(e, &q);
+        if (rc) throw Fail(rc, g_err);
+        const float td = median_step();
+        const bool keep = tl <= td;
+        if (keep) {
+            std::vector<uint8_t> kc, ks;
+            split_kinds(e, learned_kind, kc, ks);
+            build_layout(e, kc, ks, learned_p);
+        }
+        if (learned_ms) *learned_ms = tl;
+        if (degree_ms) *degree_ms = td;
+        if (kept_learned) *kept_learned = keep ? 1 : 0;
+    });
+}
+
 int lpa_time_iterations(lpa_engine *e, int method, int iters, int use_graph, float *ms, uint64_t *changed) {
     return guard([&] {
         require_ready(e);

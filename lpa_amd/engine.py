@@ -52,6 +52,8 @@ EXPORTS = {
     "lpa_mlp_loss_grad": (C.c_int, [_VP, _VP, _VP, _VP, C.c_uint64, _VP, _VP]),
     "lpa_mlp_train": (C.c_int, [_VP, _VP, _VP, _VP, C.c_uint64, C.c_int, C.c_int, C.c_float, C.c_uint64, _VP]),
     "lpa_schedule_learned": (C.c_int, [_VP, _VP]),
+    "lpa_schedule_learned_judged": (C.c_int, [_VP, _VP, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                              C.POINTER(C.c_int)]),
     "lpa_last_kernel_times": (C.c_int, [_VP, _VP]),
     "lpa_calibration_copy": (C.c_int, [_VP, C.c_uint64, C.c_int, _VP]),
     "lpa_graph_encode_csr": (C.c_int, [_VP, C.c_uint64, _VP, _VP, C.c_uint32]),
@@ -273,6 +275,14 @@ class Engine:
 
     def schedule_learned(self, mlp):
         _check(self._L.lpa_schedule_learned(self._h, mlp._h))
+
+    def schedule_learned_judged(self, mlp, iters=5):
+        """learned layout unless the degree layout's median step (iters
+        steps from the current labels, which are put back) is faster;
+        returns (kept_learned, learned_ms, degree_ms)"""
+        tl, td, k = C.c_float(0), C.c_float(0), C.c_int(0)
+        _check(self._L.lpa_schedule_learned_judged(self._h, mlp._h, iters, C.byref(tl), C.byref(td), C.byref(k)))
+        return bool(k.value), tl.value, td.value
 
     def kernel_times(self):
         t = np.zeros(6, np.float32)

@@ -62,3 +62,60 @@ def test_learned_schedule_is_bit_exact():
     assert np.isfinite(losses).all() and losses[-1] < losses[0]
     e.schedule_learned(model)
     run_and_compare(cgr, 4, K=2, cms_thresh=2000, cms_depth=4, cms_width=512, cms_seed=1, engine=e)
+
+
+def test_learned_schedule_gain_is_measured():
+    """what the trained network gains over the degree-threshold schedule it
+    replaces: both time the same K = 8 steps of R-MAT 18 (HIP events, median
+    of 5) from the same labels and must produce the same labels; the ratio is
+    printed.  The learned schedule keeps a vertex in its degree family unless
+    the model is confident, so it must not lose much either (> 0.8x)."""
+    g = L.gen_rmat(18, 16, 0.57, 0.19, 0.19, seed=1, bfs=True)
+    cgr = g.encode(0, 128)
+    cms = dict(cms_thresh=1024, cms_depth=4, cms_width=1024, cms_seed=0)
+    e = L.Engine(0)
+    e.upload(cgr)
+    e.config(K=8, **cms)
+    e.schedule_degree()
+    e.labels_init()
+    samples = L.schedule.record_samples(e)
+    model, _ = L.schedule.train_cost_model([samples], epochs=20, batch=2048)
+    out = {}
+    for name in ("degree", "learned"):
+        if name == "degree":
+            e.schedule_degree()
+        else:
+            e.schedule_learned(model)
+        e.labels_init()
+        e.nova_iterate()
+        ms, _ = e.time_iterations(5)
+        out[name] = (float(np.median(ms)), e.labels(), e.counts())
+    gain = out["degree"][0] / out["learned"][0]
+    print(f"[learned] degree {out['degree'][0]:.3f} ms, learned {out['learned'][0]:.3f} ms, "
+          f"gain {gain:.3f}x; bins degree {out['degree'][2]} learned {out['learned'][2]}", flush=True)
+    assert np.array_equal(out["degree"][1], out["learned"][1])
+    assert gain > 0.8
+    e.close()
+
+
+def test_judged_learned_schedule_keeps_labels():
+    """the degree schedule judges the learned layout: both are timed from the
+    current labels, which come back unchanged, and the kept layout gives the
+    oracle's labels"""
+    g = L.gen_rmat(14, 16, seed=22)
+    cgr = g.encode(0, 128)
+    e = L.Engine(0)
+    e.upload(cgr)
+    e.config(K=3, cms_thresh=1500, cms_depth=4, cms_width=512, cms_seed=2)
+    e.schedule_degree()
+    e.labels_init()
+    samples = L.schedule.record_samples(e)
+    model, _ = L.schedule.train_cost_model([samples], epochs=10, batch=1024)
+    e.labels_init()
+    e.nova_iterate()
+    before = e.labels()
+    kept, tl, td = e.schedule_learned_judged(model, iters=3)
+    print(f"[judge] learned {tl:.3f} ms, degree {td:.3f} ms, kept {'learned' if kept else 'degree'}", flush=True)
+    assert tl > 0 and td > 0 and kept == (tl <= td)
+    assert np.array_equal(e.labels(), before)
+    run_and_compare(cgr, 3, K=3, cms_thresh=1500, cms_depth=4, cms_width=512, cms_seed=2, engine=e)
