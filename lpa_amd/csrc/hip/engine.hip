@@ -1734,8 +1734,11 @@ int lpa_schedule_learned(lpa_engine *e, lpa_mlp *m) {
 int lpa_schedule_learned_judged(lpa_engine *e, lpa_mlp *m, int iters, float *learned_ms, float *degree_ms,
                                 int *kept_learned) {
     return guard([&] {
-        require_ready(e);
-        if (!m || iters <= 0) throw Fail(LPA_ERR_ARG, "bad arguments");
+        // builds both layouts itself: needs the graph and the labels only
+        if (!e || !m || iters <= 0) throw Fail(LPA_ERR_ARG, "bad arguments");
+        if (!e->have_graph) throw Fail(LPA_ERR_STATE, "no graph uploaded");
+        if (!e->have_labels) throw Fail(LPA_ERR_STATE, "labels not initialised");
+        HIPCHK(hipSetDevice(e->device));
         // both layouts are timed from the same labels, which are put back
         const uint64_t tot = e->n * e->K;
         std::vector<uint32_t> saved(tot);

@@ -119,3 +119,11 @@ def test_judged_learned_schedule_keeps_labels():
     assert tl > 0 and td > 0 and kept == (tl <= td)
     assert np.array_equal(e.labels(), before)
     run_and_compare(cgr, 3, K=3, cms_thresh=1500, cms_depth=4, cms_width=512, cms_seed=2, engine=e)
+    # an engine with labels but no schedule yet
+    f = L.Engine(0)
+    f.upload(cgr)
+    f.config(K=3, cms_thresh=1500, cms_depth=4, cms_width=512, cms_seed=2)
+    f.labels_init()
+    kept, tl, td = f.schedule_learned_judged(model, iters=2)
+    assert tl > 0 and td > 0
+    run_and_compare(cgr, 2, K=3, cms_thresh=1500, cms_depth=4, cms_width=512, cms_seed=2, engine=f)

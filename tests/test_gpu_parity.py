@@ -174,6 +174,16 @@ def test_heavy_local_and_chunked_paths():
     run_and_compare(g.encode(16, 128), 3, cms_thresh=1000, cms_depth=4, cms_width=512, cms_seed=8)
 
 
+@pytest.mark.parametrize("K", [5, 9])
+def test_partial_vector_batches(K):
+    """K not a multiple of the staged batch (8 vectors in the wave groups,
+    the 128 KB-table hub kernel and the CMS one-workgroup kernel, 4
+    elsewhere): the last batch is partial; exact and sketched hubs"""
+    cgr = _hub_graph(seed=23).encode(0, 128)
+    run_and_compare(cgr, 2, K=K)
+    run_and_compare(cgr, 2, K=K, cms_thresh=2500, cms_depth=4, cms_width=1024, cms_seed=4)
+
+
 def test_exact_tables_across_generation_wrap():
     """exact-mode HBM tables carry an 8-bit pass generation and are cleared
     only when it wraps (every 255 passes): after 252 steps, eight steps from
