@@ -379,9 +379,10 @@ void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
     // CMS: sketches of up to KB label vectors side by side in LDS
     const bool one = K == 1;
     const size_t cells = (size_t)cms_depth * cms_width;
-    h.kbs = (uint32_t)std::max<size_t>(
-        1, std::min<size_t>({(size_t)K, (size_t)(one ? 1 : HEAVY_KB_CMS), (size_t)(lds_max - 20480) / (cells * 4)}));
-    h.kbl = cms_local_kb(K, cells, lds_max - 20480);
+    // both CMS kernels pack two vectors' 16-bit LDS cells in a word when
+    // K > 1 (a list of <= CMS_LOCAL ids, or a chunk, cannot overflow one)
+    h.kbs = cms_local_kb(K, cells, lds_max - 20480);
+    h.kbl = h.kbs;
     if (!as_csr) {
         if (hs.cms) hipLaunchKernelGGL(k_heavy_decode<1>, dim3(hs.ntasks), dim3(BLOCK), 0, stream, h, hs.tasks.p, hs.ntasks);
         else hipLaunchKernelGGL(k_heavy_decode<0>, dim3(hs.ntasks), dim3(BLOCK), 0, stream, h, hs.tasks.p, hs.ntasks);
@@ -424,13 +425,13 @@ void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
         else hipLaunchKernelGGL(k_heavy_clear<0>, dim3(hs.nchunked), dim3(BLOCK), 0, stream, h, hs.chunked.p, hs.nchunked);
         const dim3 g(hs.nchunks), b(HBLOCK);
         if (hs.cms) {
-            const size_t sk = std::min(h.kbs, h.kb) * cells * sizeof(uint32_t);
+            const size_t sk = cms_local_lds(std::min(h.kbs, h.kb), cells);
             if (one) {
                 hipLaunchKernelGGL((k_heavy_chunks<1, 1>), g, b, sk, stream, h);
                 hipLaunchKernelGGL((k_heavy_chunks<2, 1>), g, b, 0, stream, h);
             } else {
-                hipLaunchKernelGGL((k_heavy_chunks<1, HEAVY_KB_CMS>), g, b, sk, stream, h);
-                hipLaunchKernelGGL((k_heavy_chunks<2, HEAVY_KB_CMS>), g, b, 0, stream, h);
+                hipLaunchKernelGGL((k_heavy_chunks<1, HEAVY_KB_CMS_LOCAL>), g, b, sk, stream, h);
+                hipLaunchKernelGGL((k_heavy_chunks<2, HEAVY_KB_CMS_LOCAL>), g, b, 0, stream, h);
             }
         } else {
             const size_t tb = 2 * CHUNK * sizeof(uint64_t);
@@ -586,7 +587,7 @@ int lpa_engine_create(int device, lpa_engine **out) {
             allow_group_lds<64>(dyn);
             allow_group_lds<BLOCK>(dyn);
             const void *heavy_fns[] = {(const void *)k_heavy_chunks<0, 1>, (const void *)k_heavy_chunks<0, HEAVY_KB_MAX>,
-                                       (const void *)k_heavy_chunks<1, 1>, (const void *)k_heavy_chunks<1, HEAVY_KB_CMS>,
+                                       (const void *)k_heavy_chunks<1, 1>, (const void *)k_heavy_chunks<1, HEAVY_KB_CMS_LOCAL>,
                                        (const void *)k_heavy_local<0, 1, 4>, (const void *)k_heavy_local<0, HEAVY_KB_LOCAL, 4>,
                                        (const void *)k_heavy_local<0, 1, 8>, (const void *)k_heavy_local<0, HEAVY_KB_MAX, 8>,
                                        (const void *)k_heavy_local<1, 1, 4>, (const void *)k_heavy_local<1, HEAVY_KB_CMS_LOCAL, 4>};

@@ -250,16 +250,15 @@ constexpr int HEAVY_IPL = CHUNK / HBLOCK;
 // serve up to KB label vectors (kbs sketches side by side) per round.
 constexpr int CMS_R = 4;
 // Label vectors per staged batch of the heavy kernels (template KB): 1 for
-// a single vector, else 8 for the exact chunk kernel, 4 for sketches (four
-// sketches side by side leave LDS for two workgroups per CU), 4 for the
-// exact one-workgroup kernel with 64 KB tables (16 staged labels per lane
-// fit the 64 registers of eight waves per SIMD without spilling) and 8 for
-// the one with 128 KB tables (one workgroup per CU: 128 registers).
+// a single vector, else 8 for the exact chunk kernel, 4 for the exact
+// one-workgroup kernel with 64 KB tables (16 staged labels per lane fit the
+// 64 registers of eight waves per SIMD without spilling) and 8 for the one
+// with 128 KB tables (one workgroup per CU: 128 registers).
 constexpr int HEAVY_KB_MAX = 8;
-constexpr int HEAVY_KB_CMS = 4;
 constexpr int HEAVY_KB_LOCAL = 4;
-// The one-workgroup CMS kernel packs two vectors' 16-bit cells in a word,
-// so eight sketches take the LDS of four and a row is gathered once per 8.
+// The sketch kernels (one-workgroup and chunk) pack two vectors' 16-bit LDS
+// cells in a word, so eight sketches take the LDS of four and a row is
+// gathered once per 8 vectors.
 constexpr int HEAVY_KB_CMS_LOCAL = 8;
 
 // One sketch pass over ids[0..n) for vectors k0 .. k0+kb-1 (kb <= KB;
@@ -430,6 +429,10 @@ __global__ void __launch_bounds__(HBLOCK) k_heavy_chunks(HeavyArgs h) {
                 });
         }
     } else {
+        // several vectors: the chunk-local sketches take 16-bit cells, two
+        // vectors to a word (a chunk of at most CHUNK ids cannot overflow
+        // one); the hub's HBM sketches keep 32-bit cells
+        constexpr bool PACK = KB > 1;
         const uint32_t cells = a.cms_depth << a.cms_logw;
         for (uint32_t kq = 0; kq < h.kb; kq += h.kbs) {
             const uint32_t kb = h.kb - kq < h.kbs ? h.kb - kq : h.kbs;
@@ -437,13 +440,23 @@ __global__ void __launch_bounds__(HBLOCK) k_heavy_chunks(HeavyArgs h) {
             uint64_t best[KB] = {};
             if (PASS == 1) {
                 uint32_t *lsk = (uint32_t *)lds;
-                for (uint32_t x = threadIdx.x; x < kb * cells; x += HBLOCK) lsk[x] = 0;
+                const uint32_t words = (PACK ? (kb + 1) / 2 : kb) * cells;
+                for (uint32_t x = threadIdx.x; x < words; x += HBLOCK) lsk[x] = 0;
                 __syncthreads();
-                cms_rounds<true, KB>(a, ids, n, h.k0 + kq, kb, lsk, cells, best,
-                                     KB == 1 && a.K == 1 && !h.csr ? const_cast<uint32_t *>(ids) : nullptr);
+                cms_rounds<true, KB, PACK>(a, ids, n, h.k0 + kq, kb, lsk, cells, best,
+                                           KB == 1 && a.K == 1 && !h.csr ? const_cast<uint32_t *>(ids) : nullptr);
                 __syncthreads();
-                for (uint32_t x = threadIdx.x; x < kb * cells; x += HBLOCK)
-                    if (lsk[x]) atomicAdd(&hsk[x], lsk[x]);
+                for (uint32_t x = threadIdx.x; x < words; x += HBLOCK) {
+                    const uint32_t w = lsk[x];
+                    if (!w) continue;
+                    if (PACK) {
+                        const uint32_t pair = x / cells, c = x - pair * cells;
+                        if (w & 0xFFFFu) atomicAdd(&hsk[(uint64_t)(2 * pair) * cells + c], w & 0xFFFFu);
+                        if (w >> 16) atomicAdd(&hsk[(uint64_t)(2 * pair + 1) * cells + c], w >> 16);
+                    } else {
+                        atomicAdd(&hsk[x], w);
+                    }
+                }
                 __syncthreads();
             } else {
                 cms_rounds<false, KB>(a, ids, n, h.k0 + kq, kb, hsk, cells, best,
