@@ -264,14 +264,14 @@ static int wave_groups(uint32_t logT, uint32_t quarter_max = 128) {
     return (1u << (logT - 1)) <= quarter_max ? 16 : 4;
 }
 
-// label vectors staged per row gather by the group kernels: 4 keeps the
-// staged registers low enough for four waves per SIMD, and while the label
-// rows fit the 256 MB last-level cache the second half of a 32-byte row is
-// a cache hit; beyond it every row gather costs a memory access, and 8
-// vectors per gather (one access per neighbour and step at K = 8) win.
+// label vectors staged per row gather by the group kernels: 8 from K = 5
+// (one gather per neighbour and step at K = 8; with one aggregated match
+// round the kernels stay fast enough at the lower occupancy that even label
+// rows held by the 256 MB last-level cache gain from it), else K rounded
+// down to 4 / 2 / 1.
 static int group_kb(uint32_t K, uint64_t n) {
-    const bool cached = (uint64_t)n * K * 4 <= (256ull << 20);
-    return K >= 5 && !cached ? 8 : K >= 3 ? 4 : K == 2 ? 2 : 1;
+    (void)n;
+    return K >= 5 ? 8 : K >= 3 ? 4 : K == 2 ? 2 : 1;
 }
 
 // instance of the group kernel for a batch size and staged ids per lane
