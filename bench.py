@@ -205,6 +205,7 @@ def main():
         e.config(K=K, semi=semi, **w["cms"])
         # the degree schedule judges the learned one on the same steps and
         # stays if it is faster (the learned layout's fallback)
+        e.schedule_degree()
         e.labels_init(labK)
         kept, judge_l, judge_d = e.schedule_learned_judged(model, iters=5)
         log(f"judge: learned layout {judge_l:.3f} ms, degree layout {judge_d:.3f} ms -> "
