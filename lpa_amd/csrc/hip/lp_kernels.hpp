@@ -160,13 +160,19 @@ __device__ __forceinline__ void load_row(const StepArgs &a, uint32_t u, uint32_t
 // with a bitonic network and takes the longest run.
 template <int N, bool CSR>
 __global__ void __launch_bounds__(BLOCK) k_lane(StepArgs a) {
+    // CSR rows follow the layout back to back, so the block's rows are one
+    // range of nbr: the workgroup copies it to LDS with coalesced loads
+    // instead of every lane walking its own row (N <= 32: at most 32 KB)
+    constexpr bool STAGE = CSR && N <= 32;
     __shared__ uint32_t list[CSR ? 1 : N][BLOCK];
+    __shared__ uint32_t rows[STAGE ? N * BLOCK : 1];
     __shared__ uint32_t sv[BLOCK], sd[BLOCK];
     __shared__ uint64_t sbase[CSR ? BLOCK : 1];
     const uint32_t first = a.begin + blockIdx.x * BLOCK;
     const uint32_t p = first + threadIdx.x;
     const uint32_t nact = a.end - first < (uint32_t)BLOCK ? a.end - first : (uint32_t)BLOCK;
     const uint64_t t0 = a.rec ? cycles() : 0;
+    const uint64_t b0 = STAGE ? a.row[first] : 0;
     if (p < a.end) {
         uint32_t d = 0;
         if (CSR) {
@@ -184,6 +190,12 @@ __global__ void __launch_bounds__(BLOCK) k_lane(StepArgs a) {
         sd[threadIdx.x] = d;
         if (a.rec) atomicAdd(&a.rec[sv[threadIdx.x]], cycles() - t0);
     }
+    if (STAGE) {
+        // every row of a k_lane<N> bin holds at most N ids
+        uint64_t tot = a.row[first + nact] - b0;
+        tot = tot < (uint64_t)N * BLOCK ? tot : (uint64_t)N * BLOCK;
+        for (uint32_t x = threadIdx.x; x < (uint32_t)tot; x += BLOCK) rows[x] = a.nbr[b0 + x];
+    }
     __syncthreads();
     const uint32_t K = a.K, pairs = nact * K;
     uint32_t nch = 0;
@@ -197,7 +209,9 @@ __global__ void __launch_bounds__(BLOCK) k_lane(StepArgs a) {
 #pragma unroll
         for (int j = 0; j < N; j++) {
             uint32_t u = 0;
-            if ((uint32_t)j < d) u = CSR ? a.nbr[sbase[CSR ? vb : 0] + j] : list[CSR ? 0 : j][vb];
+            if ((uint32_t)j < d)
+                u = STAGE ? rows[(uint32_t)(sbase[CSR ? vb : 0] - b0) + j]
+                    : CSR ? a.nbr[sbase[CSR ? vb : 0] + j] : list[CSR ? 0 : j][vb];
             x[j] = (uint32_t)j < d ? clean_label(a.lin[(uint64_t)u * K + k]) : UNL;
         }
         bitonic_sort<N>(x);
