@@ -7,6 +7,7 @@
  */
 #include "lpa_oracle.h"
 
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -108,7 +109,11 @@ static int put_segment(bitv *rec, uint64_t cnt, const bitv *body, uint64_t align
 
 /* encode_node (compressor.cpp:306-328) with intervalize (:330-362),
  * encode_intervals (:364-439) and encode_residuals (:441-499). */
-static int encode_record(uint64_t v, const uint32_t *adj, uint64_t d, bitv *rec) {
+/* what encode_record saw: #intervals, #residuals, #interval segments,
+ * #residual segments, sum of interval lengths (the cost-model features) */
+typedef struct { uint64_t ni, nr, si, sr, itv_sum; } rec_stats;
+
+static int encode_record(uint64_t v, const uint32_t *adj, uint64_t d, bitv *rec, rec_stats *st) {
     int rc = -1;
     uint64_t ni = 0, nr = 0;
     uint64_t *il = (uint64_t *)malloc((d / MIN_ITV + 1) * sizeof(uint64_t));
@@ -153,6 +158,13 @@ static int encode_record(uint64_t v, const uint32_t *adj, uint64_t d, bitv *rec)
                 bv_gamma(&last->bits, ilen[i] - MIN_ITV)) goto out;
         free(cur.b); cur.b = NULL; cur.len = cur.cap = 0;
     }
+    if (st) {
+        st->ni = ni;
+        st->nr = nr;
+        st->si = segs.n;
+        st->itv_sum = 0;
+        for (uint64_t i = 0; i < ni; i++) st->itv_sum += ilen[i];
+    }
     if (bv_gamma(rec, segs.n - 1)) goto out;
     for (uint64_t i = 0; i < segs.n; i++)
         if (put_segment(rec, segs.s[i].cnt, &segs.s[i].bits, i + 1 == segs.n ? 0 : SEG_BITS)) goto out;
@@ -179,6 +191,7 @@ static int encode_record(uint64_t v, const uint32_t *adj, uint64_t d, bitv *rec)
         for (uint64_t i = nr - cnt; i < nr; i++)
             if (bv_zeta(&last->bits, res[i] - res[i - 1] - 1)) goto out;
     }
+    if (st) st->sr = segs.n;
     if (bv_gamma(rec, segs.n - 1)) goto out;
     for (uint64_t i = 0; i < segs.n; i++)
         if (put_segment(rec, segs.s[i].cnt, &segs.s[i].bits, i + 1 == segs.n ? 0 : SEG_BITS)) goto out;
@@ -207,8 +220,45 @@ int orc_record_bits(uint64_t n, const uint64_t *row, const uint32_t *col, uint64
     bitv rec = {0};
     for (uint64_t v = 0; v < n; v++) {
         rec.len = 0;
-        if (encode_record(v, col + row[v], row[v + 1] - row[v], &rec)) { free(rec.b); return -1; }
+        if (encode_record(v, col + row[v], row[v + 1] - row[v], &rec, NULL)) { free(rec.b); return -1; }
         rec_bits[v] = rec.len;
+    }
+    free(rec.b);
+    return 0;
+}
+
+/* Cost-model features of every vertex (8 floats each), as the engine
+ * reports them for its compressed vertices: degree, #intervals, #residuals,
+ * record bits, #interval segments, #residual segments, mean interval length
+ * (0 without intervals), log2(1 + degree).  A vertex of degree < small_thresh
+ * is read from the small CSR: {d, 0, d, 32 d, 0, 0, 0, log2(1 + d)}.  The
+ * counts are those of the record encode_record writes; no reference
+ * counterpart (the reference has no cost model). */
+int orc_record_features(uint64_t n, const uint64_t *row, const uint32_t *col, uint32_t small_thresh,
+                        float *feat) {
+    int rc = check_rows(n, row, col);
+    if (rc) return rc;
+    bitv rec = {0};
+    for (uint64_t v = 0; v < n; v++) {
+        const uint64_t d = row[v + 1] - row[v];
+        float *f = feat + v * 8;
+        if (d < small_thresh) {
+            f[0] = (float)d; f[1] = 0.0f; f[2] = (float)d; f[3] = 32.0f * (float)d;
+            f[4] = f[5] = f[6] = 0.0f;
+            f[7] = log2f(1.0f + (float)d);
+            continue;
+        }
+        rec_stats st = {0};
+        rec.len = 0;
+        if (encode_record(v, col + row[v], d, &rec, &st)) { free(rec.b); return -1; }
+        f[0] = (float)d;
+        f[1] = (float)st.ni;
+        f[2] = (float)st.nr;
+        f[3] = (float)rec.len;
+        f[4] = (float)st.si;
+        f[5] = (float)st.sr;
+        f[6] = st.ni ? (float)st.itv_sum / (float)st.ni : 0.0f;
+        f[7] = log2f(1.0f + (float)d);
     }
     free(rec.b);
     return 0;
@@ -231,7 +281,7 @@ int orc_write_cgr(uint64_t n, const uint64_t *row, const uint32_t *col,
             int take = pass == 0 ? d >= large_thresh : (d >= small_thresh && d < large_thresh);
             if (!take) continue;
             rec.len = 0;
-            if (encode_record(v, col + row[v], d, &rec)) { free(rec.b); return -1; }
+            if (encode_record(v, col + row[v], d, &rec, NULL)) { free(rec.b); return -1; }
             for (uint64_t i = 0; i < rec.len; i++, nbits++) {
                 if ((nbits >> 3) >= stream_cap) { free(rec.b); return -6; }
                 if (rec.b[i]) stream[nbits >> 3] |= (uint8_t)(0x80u >> (nbits & 7));

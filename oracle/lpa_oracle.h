@@ -52,6 +52,13 @@ uint32_t orc_zeta_len(uint64_t x);
 int orc_record_bits(uint64_t n, const uint64_t *row, const uint32_t *col,
                     uint64_t *rec_bits);
 
+/* Cost-model features, 8 floats per vertex (no reference counterpart):
+ * degree, #intervals, #residuals, record bits, #interval segments,
+ * #residual segments, mean interval length, log2(1 + degree); vertices of
+ * degree < small_thresh (small CSR) get {d, 0, d, 32 d, 0, 0, 0, log2(1+d)}. */
+int orc_record_features(uint64_t n, const uint64_t *row, const uint32_t *col,
+                        uint32_t small_thresh, float *feat);
+
 /* Write the reference file set in memory (compressor.cpp:178-290):
  *   stream      MSB-first bytes, ceil(nbits/8) of them (caller zeroes it),
  *   end_offs    n_c cumulative END bit offsets (the .offset lines after the count),

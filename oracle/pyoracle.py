@@ -40,6 +40,9 @@ def lib():
         L.orc_zeta_len.argtypes = [C.c_uint64]
         L.orc_record_bits.restype = C.c_int
         L.orc_record_bits.argtypes = [C.c_uint64, U64P, U32P, U64P]
+        L.orc_record_features.restype = C.c_int
+        L.orc_record_features.argtypes = [C.c_uint64, U64P, U32P, C.c_uint32,
+                                          np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")]
         L.orc_write_cgr.restype = C.c_int
         L.orc_write_cgr.argtypes = [C.c_uint64, U64P, U32P, C.c_uint32, C.c_uint32,
                                     U8P, C.c_uint64, U64P, U32P, U32P, U32P, U64P]
@@ -86,6 +89,21 @@ def record_bits(row, col):
     rc = lib().orc_record_bits(n, row, col, out)
     if rc:
         raise ValueError(f"orc_record_bits failed: {rc}")
+    return out
+
+
+def record_features(row, col, small_thresh=0):
+    """(n, 8) float32 cost-model features per vertex id: degree, #intervals,
+    #residuals, record bits, #interval segments, #residual segments, mean
+    interval length, log2(1 + degree) (small-CSR vertices: d, 0, d, 32 d, 0,
+    0, 0, log2(1 + d))"""
+    row = np.ascontiguousarray(row, np.uint64)
+    col = np.ascontiguousarray(col, np.uint32)
+    n = len(row) - 1
+    out = np.zeros((n, 8), np.float32)
+    rc = lib().orc_record_features(n, row, col, small_thresh, out)
+    if rc:
+        raise ValueError(f"orc_record_features failed: {rc}")
     return out
 
 

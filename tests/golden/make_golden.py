@@ -133,8 +133,11 @@ CASES = {
 
 
 def main():
+    # the reference compressor is built here, from its sources under
+    # /root/reference, into oracle/_ref (git- and gpurun-ignored)
+    subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
     if not os.path.exists(REF_BIN):
-        sys.exit("build the reference compressor first: make -C oracle ref")
+        sys.exit("the reference compressor did not build (oracle/Makefile `ref`)")
     for name, (fn, thresholds) in CASES.items():
         edges, one_way = fn()
         for t in thresholds:
