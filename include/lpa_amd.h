@@ -212,11 +212,12 @@ int lpa_mlp_train(lpa_mlp *m, const float *x, const float *target, const uint8_t
  * the wave family is offered up to degree 512; vertices of the small CSR
  * part keep their degree family. */
 int lpa_schedule_learned(lpa_engine *e, lpa_mlp *m);
-/* The learned schedule judged by the degree schedule it replaces: both
- * layouts run `iters` fused steps from the current labels (put back
- * afterwards); the learned layout is kept unless the degree layout's median
- * step is faster, in which case the degree layout stays (fallback).  Reports
- * both medians (ms) and which was kept. */
+/* The learned schedule judged by the degree schedule it replaces: the
+ * degree layout, the learned layout and the degree layout again each run
+ * `iters` fused steps from the current labels (put back afterwards); the
+ * learned layout is kept unless the degree layout's median step (over both
+ * of its runs) is faster, in which case the degree layout stays (fallback).
+ * Reports both medians (ms) and which was kept. */
 int lpa_schedule_learned_judged(lpa_engine *e, lpa_mlp *m, int iters, float *learned_ms, float *degree_ms,
                                 int *kept_learned);
 

@@ -92,11 +92,30 @@ File::File(const std::string _file_name, bool _compress, bool _one_way, int sms)
     : small_node_thresh(sms), file_name(_file_name), compress(_compress), one_way(_one_way),
       has_small(sms != 0) {}
 
-bool File::cache_matches() const {
+int File::cache_state() const {
     std::ifstream f(file_name + ".lpa_params");
+    if (!f) return 0;
     int t = -1, o = -1, c = -1;
-    if (!(f >> t >> o >> c)) return false;
-    return t == small_node_thresh && o == (one_way ? 1 : 0) && c == (compress ? 1 : 0);
+    if (!(f >> t >> o >> c)) return -1;
+    return t == small_node_thresh && o == (one_way ? 1 : 0) && c == (compress ? 1 : 0) ? 1 : -1;
+}
+
+bool File::cache_matches() const { return cache_state() == 1; }
+
+// A file set with no parameter record (compressed runs only: an
+// uncompressed reference set has no .graph, which this loader reads): its
+// .offset count must be the number of vertices of degree >= the threshold in
+// .row, else the set was written for another threshold.
+static bool reference_set_consistent(const std::string &base, bool compress, int thresh) {
+    if (!compress) return false;
+    std::vector<uint32_t> row;
+    read_vec(base + ".row", row);
+    std::ifstream f(base + ".offset");
+    uint64_t nc = 0;
+    if (row.empty() || !(f >> nc)) return false;
+    uint64_t want = 0;
+    for (size_t v = 0; v + 1 < row.size(); v++) want += row[v + 1] - row[v] >= (uint32_t)thresh;
+    return nc == want;
 }
 
 void File::write_binary() {
@@ -112,13 +131,15 @@ void File::write_binary() {
 }
 
 void File::load_graph() {
-    const bool have = exists(file_name + ".graph") && exists(file_name + ".offset") &&
-                      exists(file_name + ".row") && exists(file_name + ".clm");
-    if (!have || !cache_matches()) write_binary();
+    // the files File::load_graph reads (file.cpp:29-60)
+    const bool have = exists(file_name + ".row") && exists(file_name + ".clm") && exists(file_name + ".graph") &&
+                      exists(file_name + ".offset") && exists(file_name + ".vertices.bin");
+    const int st = cache_state();
+    const int t_req = !compress ? 1 : (has_small ? small_node_thresh : 0);
+    if (!have || st < 0 || (st == 0 && !reference_set_consistent(file_name, compress, t_req))) write_binary();
     lpah_cgr *c = nullptr;
     // uncompressed runs keep every vertex in the small CSR
-    const int t = !compress ? 1 : (has_small ? small_node_thresh : 0);
-    check(lpah_files_load(file_name.c_str(), t, &c), "load");
+    check(lpah_files_load(file_name.c_str(), t_req, &c), "load");
     {
         const uint64_t nw = lpah_cgr_nwords(c);
         graph.assign(lpah_cgr_words(c), lpah_cgr_words(c) + nw);

@@ -319,8 +319,11 @@ __global__ void __launch_bounds__(256) k_scan_top(uint64_t *__restrict__ sums, u
 }
 
 // out[i] = base of the block + exclusive scan inside it; out[n] = total.
-__global__ void __launch_bounds__(256) k_scan_apply(const uint64_t *__restrict__ in, uint64_t n,
-                                                    const uint64_t *__restrict__ sums, uint64_t *__restrict__ out,
+// out may be in (an in-place scan): each lane reads its 16 values before the
+// block's barrier and writes the same 16 after it, so in and out are not
+// declared restrict.
+__global__ void __launch_bounds__(256) k_scan_apply(const uint64_t *in, uint64_t n,
+                                                    const uint64_t *__restrict__ sums, uint64_t *out,
                                                     uint64_t nb) {
     __shared__ uint64_t sh[4];
     const uint64_t b0 = (uint64_t)blockIdx.x * SCAN_PER_BLOCK;

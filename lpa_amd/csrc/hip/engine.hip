@@ -1746,24 +1746,34 @@ int lpa_schedule_learned_judged(lpa_engine *e, lpa_mlp *m, int iters, float *lea
         HIPCHK(hipMemcpy(saved.data(), e->L[e->cur].p, tot * 4, hipMemcpyDeviceToHost));
         const int saved_cur = e->cur;
         std::vector<float> ms(iters);
-        auto median_step = [&]() {
+        // steps of one layout, from the saved labels (put back afterwards)
+        auto run_steps = [&](std::vector<float> &all) {
             int rc = lpa_time_iterations(e, 0, iters, 0, ms.data(), nullptr);
             if (rc) throw Fail(rc, g_err);
             e->cur = saved_cur;
             HIPCHK(hipMemcpy(e->L[e->cur].p, saved.data(), tot * 4, hipMemcpyHostToDevice));
-            std::vector<float> sorted(ms);
-            std::sort(sorted.begin(), sorted.end());
-            return sorted[sorted.size() / 2];
+            all.insert(all.end(), ms.begin(), ms.end());
         };
-        int rc = lpa_schedule_learned(e, m);
+        auto median = [](std::vector<float> v) {
+            std::sort(v.begin(), v.end());
+            return v[v.size() / 2];
+        };
+        // order degree, learned, degree: neither layout is always measured
+        // first, and the degree layout's median covers both ends
+        std::vector<float> dl, ll;
+        lpa_sched_params q{32, 512, 0, 1, 128};
+        int rc = lpa_schedule_degree(e, &q);
         if (rc) throw Fail(rc, g_err);
-        const float tl = median_step();
+        run_steps(dl);
+        rc = lpa_schedule_learned(e, m);
+        if (rc) throw Fail(rc, g_err);
+        run_steps(ll);
         const std::vector<uint8_t> learned_kind = e->kind_of;
         const lpa_sched_params learned_p = e->sched_used;
-        lpa_sched_params q{32, 512, 0, 1, 128};
         rc = lpa_schedule_degree(e, &q);
         if (rc) throw Fail(rc, g_err);
-        const float td = median_step();
+        run_steps(dl);
+        const float tl = median(ll), td = median(dl);
         const bool keep = tl <= td;
         if (keep) {
             std::vector<uint8_t> kc, ks;

@@ -9,7 +9,10 @@
 //    (the reference encodes a duplicate as a gap of -1, compressor.cpp:459).
 //  * The preprocessing cache records its threshold and direction in
 //    <path>.lpa_params and is rebuilt when they differ (the reference reuses
-//    a stale cache, file.cpp:29-52).
+//    a stale cache, file.cpp:29-52).  A file set without that record (one
+//    the reference wrote) is loaded as the reference loads it, without the
+//    edge list, once its compressed-vertex count is found consistent with the
+//    requested threshold; an inconsistent one is rebuilt from the edge list.
 //  * CgrCompressor supports the parameters File uses (zeta 3, minimum
 //    interval 4, 256-bit segments); other values throw std::invalid_argument.
 #pragma once
@@ -69,6 +72,7 @@ class File {
     bool has_small;
     void write_binary();
     bool cache_matches() const;
+    int cache_state() const;   // 1: matches, 0: absent, -1: differs
 };
 
 }  // namespace lpa_api

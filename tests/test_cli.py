@@ -61,3 +61,36 @@ def test_driver_labels_match_oracle(prog, tmp_path):
     lines = open(out).read().splitlines()
     assert lines[0] == f"{prog} file: {src}" and lines[1].startswith(" running time: ") \
         and lines[1].endswith("thre: 32") and lines[2] == "~~~~~"
+
+
+@pytest.mark.parametrize("case", ["mixed400_t32", "multiseg6000_t0"])
+def test_reference_file_set_loads_without_edge_list(case, tmp_path):
+    """a file set the reference wrote (no .lpa_params, no edge list) is
+    loaded as the reference loads it (src/file.cpp:29-60), not rebuilt"""
+    g = load_golden(case)
+    ref = os.path.join(GOLDEN, case, "edges.txt")
+    src = str(tmp_path / "edges.txt")
+    for suf in (".graph", ".offset", ".vertices.bin", ".row", ".clm", ".graph.small.bin", ".offset.small.bin"):
+        if os.path.exists(ref + suf):
+            shutil.copy(ref + suf, src + suf)
+    t = g["params"]["small_thresh"]
+    r = subprocess.run([os.path.join(BIN, "nova"), src, "1", str(g["params"]["one_way"]), str(t),
+                        str(tmp_path / "out.txt"), "--iters", "1"], capture_output=True, text=True, timeout=120)
+    assert f"clm_size: {len(g['clm'])} row_size: {g['n'] + 1}" in r.stdout, (r.stdout, r.stderr)
+    assert f"vertices: {g['n']}" in r.stdout
+    assert not os.path.exists(src + ".lpa_params")
+    assert open(src + ".graph", "rb").read() == open(ref + ".graph", "rb").read()
+
+
+def test_reference_file_set_of_another_threshold_is_rebuilt(tmp_path):
+    """asked for threshold 32 on a set written with threshold 0 and no edge
+    list: the set is inconsistent, the rebuild needs the edge list, and the
+    driver stops instead of running on the wrong split"""
+    ref = os.path.join(GOLDEN, "mixed400_t0", "edges.txt")
+    src = str(tmp_path / "edges.txt")
+    for suf in (".graph", ".offset", ".vertices.bin", ".row", ".clm"):
+        shutil.copy(ref + suf, src + suf)
+    r = subprocess.run([os.path.join(BIN, "nova"), src, "1", "1", "32", str(tmp_path / "out.txt"), "--iters", "1"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0 and "clm_size" not in r.stdout
+    assert "cannot open" in r.stderr

@@ -1,15 +1,20 @@
 """Parity at the benchmark workloads' full sizes, through the C ABI.
 
-R-MAT scale 22 (the default benchmark graph, 4.2 M vertices, 128 M arcs)
-runs by default: ten steps with K = 1 and two with K = 8 (count-min sketch
-4 x 1024 above degree 4096), bit for bit against the oracle after every
-step.  R-MAT scale 27 (K = 8, 8.6 G arcs) and scale 24 (K = 64,
-semi-supervised) take minutes to generate on the host; with
-LPA_FULL_SCALE=1 their one-step checks run: the oracle recomputes L_{t+1}
-from the engine's L_t for a random sample of a million vertices, after the
-first and after the last step.
+R-MAT scale 22 (4.2 M vertices, 128 M arcs): ten steps with K = 1 and two
+with K = 8 (count-min sketch 4 x 1024 above degree 4096), bit for bit
+against the oracle's own trajectory after every step.
+
+R-MAT scale 27 (134 M vertices, 8.36 G arcs, K = 8, count-min sketch 4 x 2048
+above degree 8192; ten steps) and R-MAT scale 24 (16.8 M vertices, K = 64
+semi-supervised vectors with 1 % seeds from 100 classes; fifteen steps) are
+too large for the oracle to follow a whole trajectory in minutes, so they
+take one-step checks: after a step the oracle recomputes L_{t+1} from the
+engine's own L_t.  Every vertex and every vector is recomputed after the
+first and the last step; a random sample of vertices after the others.
+When each step passes for every vertex, the whole trajectory is the
+oracle's.
 """
-import os
+import time
 
 import numpy as np
 import pytest
@@ -20,8 +25,11 @@ from gpu_common import make_engine, oracle_view, run_and_compare
 
 pytestmark = pytest.mark.gpu
 
-FULL = os.environ.get("LPA_FULL_SCALE") == "1"
 CMS22 = dict(cms_thresh=4096, cms_depth=4, cms_width=1024, cms_seed=0)
+
+
+def log(msg):
+    print(f"[scale] {msg}", flush=True)
 
 
 @pytest.fixture(scope="module")
@@ -45,46 +53,62 @@ def test_rmat20_k64_uncached_rows():
     run_and_compare(g.encode(0, 128), 2, K=64, cms_thresh=3000, cms_depth=4, cms_width=1024, cms_seed=4)
 
 
-def one_step_checks(cgr, iters, check_at, K, semi=False, labels0=None, sample=1_000_000, **cms):
-    """runs `iters` engine steps; after each step in check_at, recomputes a
-    sample of vertices with the oracle from the engine's previous labels"""
+def one_step_checks(cgr, iters, full_at, K, semi=False, labels0=None, sample=200_000, **cms):
+    """runs `iters` engine steps; after every step the oracle recomputes the
+    step from the engine's previous labels: all vertices after the steps in
+    full_at (0-based), a random sample of vertices after the others"""
     e = make_engine(cgr, K=K, semi=semi, **cms)
     e.labels_init(labels0)
     view = oracle_view(cgr)
     c = O.cfg(K=K, semi=semi, **cms)
     seeds = None
     if semi:
-        lab0 = np.ascontiguousarray(labels0, np.uint32)
-        seeds = (lab0 != O.UNLABELLED).astype(np.uint8)
+        seeds = (np.ascontiguousarray(labels0, np.uint32) != O.UNLABELLED).astype(np.uint8)
     rng = np.random.default_rng(7)
-    npos = cgr.n_c + len(cgr.small_offsets) - 1 if len(cgr.small_offsets) else cgr.n_c
+    npos = cgr.n
+    before = e.labels()
+    checked = 0
     for it in range(iters):
-        before = e.labels() if it in check_at else None
-        e.nova_iterate()
-        if before is None:
-            continue
+        ch = e.nova_iterate()
         after = e.labels()
-        pos = np.sort(rng.choice(npos, min(sample, npos), replace=False)).astype(np.uint64)
-        out = before.copy()
-        O.step_cgr_subset(view, before, pos, out, c, seeds)
-        verts = cgr.vertices[pos.astype(np.int64)].astype(np.int64)
-        idx = (verts[:, None] * K + np.arange(K)[None, :]).reshape(-1)
-        bad = np.nonzero(after[idx] != out[idx])[0]
-        print(f"[scale] step {it + 1}: {len(idx)} sampled labels, {len(bad)} differ", flush=True)
-        assert len(bad) == 0, (it, idx[bad[:5]], after[idx[bad[:5]]], out[idx[bad[:5]]])
+        t0 = time.perf_counter()
+        if it in full_at:
+            want, och = O.step_cgr(view, before, c, seeds)
+            bad = np.nonzero(after != want)[0]
+            assert ch == och, (it, ch, och)
+            n_lab = len(want)
+        else:
+            pos = np.sort(rng.choice(npos, min(sample, npos), replace=False)).astype(np.uint64)
+            want = before.copy()
+            O.step_cgr_subset(view, before, pos, want, c, seeds)
+            verts = cgr.vertices[pos.astype(np.int64)].astype(np.int64)
+            idx = (verts[:, None] * K + np.arange(K)[None, :]).reshape(-1)
+            bad = idx[np.nonzero(after[idx] != want[idx])[0]]
+            n_lab = len(idx)
+        checked += n_lab
+        log(f"step {it + 1}: {'all' if it in full_at else 'sampled'} {n_lab} labels checked in "
+            f"{time.perf_counter() - t0:.1f} s, {len(bad)} differ, {ch} changed")
+        assert len(bad) == 0, (it, bad[:5], after[bad[:5]], want[bad[:5]])
+        before = after
     e.close()
+    return checked
 
 
-@pytest.mark.skipif(not FULL, reason="R-MAT scale 27 takes minutes to build; set LPA_FULL_SCALE=1")
 def test_rmat27_k8_one_step_checks():
+    """R-MAT scale 27, edge factor 32, seed 4, BFS order; K = 8 from L0 = v;
+    CMS 4 x 2048 above degree 8192; ten steps"""
+    t0 = time.perf_counter()
     g = L.gen_rmat(27, 32, 0.57, 0.19, 0.19, seed=4, bfs=True)
     cgr = g.encode(0, 128)
+    log(f"R-MAT 27: n={g.n} m={g.m} bits={cgr.nbits} built in {time.perf_counter() - t0:.0f} s")
     del g
     one_step_checks(cgr, 10, (0, 9), 8, cms_thresh=8192, cms_depth=4, cms_width=2048, cms_seed=0)
 
 
-@pytest.mark.skipif(not FULL, reason="R-MAT scale 24 with K = 64 takes minutes; set LPA_FULL_SCALE=1")
 def test_rmat24_k64_semi_one_step_checks():
+    """R-MAT scale 24, edge factor 16, seed 3, BFS order; K = 64 vectors, each
+    with 1 % of the vertices seeded from 100 classes (clamped), the rest
+    unlabelled; exact mode; fifteen steps"""
     g = L.gen_rmat(24, 16, 0.57, 0.19, 0.19, seed=3, bfs=True)
     cgr = g.encode(0, 128)
     n, K = g.n, 64
@@ -94,4 +118,4 @@ def test_rmat24_k64_semi_one_step_checks():
     for k in range(K):
         idx = rng.choice(n, n // 100, replace=False)
         lab[idx, k] = rng.integers(0, 100, len(idx), dtype=np.uint32)
-    one_step_checks(cgr, 15, (0, 14), K, semi=True, labels0=lab.reshape(-1), sample=200_000)
+    one_step_checks(cgr, 15, (0, 14), K, semi=True, labels0=lab.reshape(-1), sample=20_000)
