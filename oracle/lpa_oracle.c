@@ -3,7 +3,7 @@
  *
  * Written for obviousness, not speed: the encoder keeps one byte per bit,
  * the decoder reads codes through a bounds-checked 64-bit window, and the
- * mode is found by sorting each neighbourhood's labels.
+ * mode is found by sorting each neighbourhood's labels (radix sort).
  */
 #include "lpa_oracle.h"
 
@@ -415,26 +415,37 @@ static uint64_t cms_col(const uint64_t *ab, uint32_t r, uint32_t lw, uint32_t x)
     return lw ? h >> (64 - lw) : 0;
 }
 
-static int cmp_u32(const void *a, const void *b) {
-    uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
-    return x < y ? -1 : x > y;
-}
-static void sort_u32(uint32_t *a, uint64_t n) {
-    if (n <= 32) {
+/* Ascending sort of n labels: insertion sort for short lists, else an LSD
+ * radix sort over bytes (tmp holds n entries); byte passes whose digit is
+ * the same in every label are skipped. */
+static void sort_u32(uint32_t *a, uint64_t n, uint32_t *tmp) {
+    if (n <= 48) {
         for (uint64_t i = 1; i < n; i++) {
             uint32_t x = a[i];
             uint64_t j = i;
             while (j > 0 && a[j - 1] > x) { a[j] = a[j - 1]; j--; }
             a[j] = x;
         }
-    } else {
-        qsort(a, n, sizeof(uint32_t), cmp_u32);
+        return;
     }
+    uint32_t or_all = 0, and_all = 0xFFFFFFFFu;
+    for (uint64_t i = 0; i < n; i++) { or_all |= a[i]; and_all &= a[i]; }
+    uint32_t *src = a, *dst = tmp;
+    for (int sh = 0; sh < 32; sh += 8) {
+        if ((((or_all ^ and_all) >> sh) & 0xFFu) == 0) continue;   /* constant digit */
+        uint64_t cnt[257] = {0};
+        for (uint64_t i = 0; i < n; i++) cnt[((src[i] >> sh) & 0xFFu) + 1]++;
+        for (int b = 0; b < 256; b++) cnt[b + 1] += cnt[b];
+        for (uint64_t i = 0; i < n; i++) dst[cnt[(src[i] >> sh) & 0xFFu]++] = src[i];
+        uint32_t *t = src; src = dst; dst = t;
+    }
+    if (src != a) memcpy(a, src, n * sizeof(uint32_t));
 }
 
 typedef struct {
     uint32_t *nbr; uint64_t nbr_cap;
-    uint32_t *lab; uint64_t lab_cap;
+    uint32_t *lab; uint64_t lab_cap;   /* K x d gathered labels, vector-major */
+    uint32_t *tmp; uint64_t tmp_cap;   /* radix-sort buffer */
     uint32_t *sketch;
 } scratch_t;
 
@@ -448,17 +459,15 @@ static int grow(uint32_t **p, uint64_t *cap, uint64_t need) {
     return 0;
 }
 
-/* f over the neighbourhood nbr[0..d) of vertex v for vector k (LP-SC):
- * most frequent label, ties to the smallest; unlabelled entries ignored;
- * no labelled neighbour -> keep; CMS estimate when d > cms_thresh. */
-static uint32_t mode_of(const orc_lp_cfg *c, const uint64_t *ab, scratch_t *s,
-                        const uint32_t *nbr, uint64_t d, uint32_t k,
-                        const uint32_t *in, uint32_t old) {
-    uint64_t K = c->K, m = 0;
-    for (uint64_t j = 0; j < d; j++) {
-        uint32_t l = in[(uint64_t)nbr[j] * K + k];
-        if (l != ORC_UNLABELLED) s->lab[m++] = l;
-    }
+/* f over the labels lab[0..d) of one vertex's neighbours for one vector
+ * (LP-SC): most frequent label, ties to the smallest; unlabelled entries
+ * ignored; no labelled neighbour -> keep; CMS estimate when d > cms_thresh.
+ * lab is reordered. */
+static uint32_t mode_of(const orc_lp_cfg *c, const uint64_t *ab, scratch_t *s, uint32_t *lab, uint64_t d,
+                        uint32_t old) {
+    uint64_t m = 0;
+    for (uint64_t j = 0; j < d; j++)
+        if (lab[j] != ORC_UNLABELLED) lab[m++] = lab[j];
     if (m == 0) return old;
     if (c->cms_thresh && d > c->cms_thresh) {
         uint32_t w = c->cms_width, lw = 0;
@@ -466,24 +475,24 @@ static uint32_t mode_of(const orc_lp_cfg *c, const uint64_t *ab, scratch_t *s,
         memset(s->sketch, 0, (size_t)c->cms_depth * w * sizeof(uint32_t));
         for (uint64_t j = 0; j < m; j++)
             for (uint32_t r = 0; r < c->cms_depth; r++)
-                s->sketch[(uint64_t)r * w + cms_col(ab, r, lw, s->lab[j])]++;
+                s->sketch[(uint64_t)r * w + cms_col(ab, r, lw, lab[j])]++;
         uint32_t best = 0, best_est = 0;
         for (uint64_t j = 0; j < m; j++) {
             uint32_t est = 0xFFFFFFFFu;
             for (uint32_t r = 0; r < c->cms_depth; r++) {
-                uint32_t e = s->sketch[(uint64_t)r * w + cms_col(ab, r, lw, s->lab[j])];
+                uint32_t e = s->sketch[(uint64_t)r * w + cms_col(ab, r, lw, lab[j])];
                 if (e < est) est = e;
             }
-            if (est > best_est || (est == best_est && s->lab[j] < best)) { best_est = est; best = s->lab[j]; }
+            if (est > best_est || (est == best_est && lab[j] < best)) { best_est = est; best = lab[j]; }
         }
         return best;
     }
-    sort_u32(s->lab, m);
-    uint32_t best = s->lab[0];
+    sort_u32(lab, m, s->tmp);
+    uint32_t best = lab[0];
     uint64_t best_run = 0, run = 0;
     for (uint64_t j = 0; j < m; j++) {
-        run = (j > 0 && s->lab[j] == s->lab[j - 1]) ? run + 1 : 1;
-        if (run > best_run) { best_run = run; best = s->lab[j]; }   /* strict: first (smallest) wins */
+        run = (j > 0 && lab[j] == lab[j - 1]) ? run + 1 : 1;
+        if (run > best_run) { best_run = run; best = lab[j]; }   /* strict: first (smallest) wins */
     }
     return best;
 }
@@ -549,17 +558,26 @@ static int64_t step_generic(const gview *g, const orc_lp_cfg *cfg, const uint8_t
             for (uint64_t j = 0; j < (uint64_t)d; j++)
                 if (s.nbr[j] >= g->n) { err = -4; break; }
             if (err) continue;
-            if (grow(&s.lab, &s.lab_cap, (uint64_t)d + 1)) { err = -9; continue; }
+            const uint64_t K = cfg->K;
+            if (grow(&s.lab, &s.lab_cap, K * (uint64_t)d + 1) || grow(&s.tmp, &s.tmp_cap, (uint64_t)d + 1)) {
+                err = -9;
+                continue;
+            }
+            /* a neighbour's K labels are one row: gathered once, vector-major */
+            for (uint64_t j = 0; j < (uint64_t)d; j++) {
+                const uint32_t *rowp = in + (uint64_t)s.nbr[j] * K;
+                for (uint64_t k = 0; k < K; k++) s.lab[k * (uint64_t)d + j] = rowp[k];
+            }
             for (uint32_t k = 0; k < cfg->K; k++) {
                 uint64_t idx = (uint64_t)v * cfg->K + k;
                 uint32_t old = in[idx], nl;
                 if (cfg->semi && seeds && seeds[idx]) nl = old;
-                else nl = mode_of(cfg, ab, &s, s.nbr, (uint64_t)d, k, in, old);
+                else nl = mode_of(cfg, ab, &s, s.lab + (uint64_t)k * d, (uint64_t)d, old);
                 out[idx] = nl;
                 changed += nl != old;
             }
         }
-        free(s.nbr); free(s.lab); free(s.sketch);
+        free(s.nbr); free(s.lab); free(s.tmp); free(s.sketch);
     }
     return err ? err : changed;
 }
