@@ -120,6 +120,25 @@ int lpa_graph_upload(lpa_engine *e, uint64_t n, uint64_t n_c,
  * duplicate or out-of-range ids (LPA_ERR_FORMAT). */
 int lpa_graph_encode_csr(lpa_engine *e, uint64_t n, const uint64_t *row, const uint32_t *col,
                          uint32_t small_thresh);
+/* Allocation-free variant of lpa_graph_encode_csr.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * All arrays are DEVICE pointers the caller owns: row[n+1], col[row[n]]
+ * (every row strictly increasing), vid[n_c] the vertex of each compressed
+ * position (the stream order, e.g. deg >= 128 by id, then the rest by id);
+ * outputs offsets[n_c+1] (start bit of each record, offsets[n_c] = total)
+ * and words[words_cap] (64-bit words, bit 0 = MSB of word 0; the first
+ * ceil(total/64) are written).  *nbits (host) receives the total.  The same
+ * kernels as lpa_graph_encode_csr write the same bits; nothing is allocated:
+ * the scan's block sums and the error flag live in a 1 MiB device array of
+ * the library (one per GPU, kept until the process ends), which limits n_c
+ * to 536,866,816 records (LPA_ERR_RANGE beyond).  words_cap too small ->
+ * LPA_ERR_RANGE with words untouched; a bad row -> LPA_ERR_FORMAT.  Runs on
+ * the engine's stream and returns when done; calls that overlap on one GPU
+ * (several threads or engines) race on the shared array. */
+int lpa_encode_csr_device(lpa_engine *e, uint64_t n, uint64_t n_c, const uint64_t *row, const uint32_t *col,
+                          const uint32_t *vid, uint64_t *offsets, uint64_t *words, uint64_t words_cap,
+                          uint64_t *nbits);
 /* The engine's compressed part as stored: *n_c, *nbits, and when not NULL
  * words[ceil(nbits/64)] (64-bit words, bit 0 = MSB of word 0), offsets[n_c+1]
  * (start bit of each record) and vertices[n] (stream order, then small) --

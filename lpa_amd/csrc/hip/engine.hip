@@ -18,6 +18,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -135,9 +136,38 @@ uint64_t splitmix64(uint64_t &s) {
 
 }  // namespace
 
+// Device scratch of the allocation-free encoder (lpa_encode_csr_device): word
+// 0 is the error flag, words 1.. the scan's block sums.  One copy per GPU,
+// held until the process ends; its address on each device comes from
+// hipGetSymbolAddress (the host-side name is only a stand-in).
+constexpr uint64_t NOALLOC_SCRATCH_WORDS = 1ull << 17;   // 1 MiB
+constexpr uint64_t NOALLOC_MAX_NC = (NOALLOC_SCRATCH_WORDS - 1) * SCAN_PER_BLOCK;
+static __device__ __align__(256) uint64_t scratch[NOALLOC_SCRATCH_WORDS];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+uint64_t *g_scratch_ptr[MAX_DEVICES] = {};
+
+// The scratch address on `device`, resolved once per device; the device
+// must be current.
+uint64_t *device_scratch(int device) {
+    if (device < 0 || device >= MAX_DEVICES) throw Fail(LPA_ERR_RANGE, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) throw Fail(LPA_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[device] = static_cast<uint64_t *>(ptr);
+    }
+    return g_scratch_ptr[device];
+}
+}  // namespace
+
 struct lpa_engine {
     int device = 0;
     hipStream_t stream = nullptr;
+    uint64_t *scratch_dev = nullptr;   // this device's copy of `scratch`
     int num_cu = 0;
     size_t lds_max = 65536;
 
@@ -579,6 +609,7 @@ int lpa_engine_create(int device, lpa_engine **out) {
             e->num_cu = prop.multiProcessorCount;
             e->lds_max = prop.sharedMemPerBlock;
             HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+            e->scratch_dev = device_scratch(device);
             e->counter.alloc(1);
             HIPCHK(hipHostMalloc((void **)&e->h_counter, sizeof(unsigned long long), hipHostMallocDefault));
             const int dyn = (int)(e->lds_max - 20480);
@@ -752,6 +783,59 @@ int lpa_graph_encode_csr(lpa_engine *e, uint64_t n, const uint64_t *row, const u
                                dcol.p, e->vid0.p, n_c, e->offs0.p, e->words0.p);
         HIPCHK(hipGetLastError());
         finish_graph(e);
+    });
+}
+
+int lpa_encode_csr_device(lpa_engine *e, uint64_t n, uint64_t n_c, const uint64_t *row, const uint32_t *col,
+                          const uint32_t *vid, uint64_t *offsets, uint64_t *words, uint64_t words_cap,
+                          uint64_t *nbits) {
+    return guard([&] {
+        if (!e || !row || !vid || !offsets || !words || !nbits) throw Fail(LPA_ERR_ARG, "null argument");
+        if (n >= (1ull << 30)) throw Fail(LPA_ERR_RANGE, "n must be < 2^30");
+        if (n_c > n) throw Fail(LPA_ERR_ARG, "n_c > n");
+        if (n_c > NOALLOC_MAX_NC)
+            throw Fail(LPA_ERR_RANGE, "n_c exceeds the allocation-free encoder's limit of " +
+                                          std::to_string(NOALLOC_MAX_NC) + " records");
+        HIPCHK(hipSetDevice(e->device));
+        uint64_t *scr = e->scratch_dev;   // [0] error flag, [1..nb] block sums
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, e->stream, (unsigned long long *)scr, (uint64_t)1, 0ull);
+        if (n_c == 0) {
+            hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, e->stream, (unsigned long long *)offsets, (uint64_t)1,
+                               0ull);
+            HIPCHK(hipGetLastError());
+            e->sync();
+            *nbits = 0;
+            return;
+        }
+        // record lengths into offsets[0..n_c), then an in-place exclusive scan
+        hipLaunchKernelGGL(k_encode_len, dim3((unsigned)((n_c + 255) / 256)), dim3(256), 0, e->stream, row, col, vid,
+                           n_c, (uint32_t)n, offsets, (unsigned long long *)scr);
+        const uint64_t nb = (n_c + SCAN_PER_BLOCK - 1) / SCAN_PER_BLOCK;
+        hipLaunchKernelGGL(k_scan_sums, dim3((unsigned)nb), dim3(256), 0, e->stream, offsets, n_c, scr + 1);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, e->stream, scr + 1, nb);
+        hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(256), 0, e->stream, offsets, n_c, scr + 1, offsets,
+                           nb);
+        HIPCHK(hipGetLastError());
+        // error flag and total length to the host (the engine's pinned word)
+        HIPCHK(hipMemcpyAsync(e->h_counter, scr, sizeof(uint64_t), hipMemcpyDeviceToHost, e->stream));
+        e->sync();
+        if (*e->h_counter)
+            throw Fail(LPA_ERR_FORMAT, "row of vertex " + std::to_string(*e->h_counter - 1) +
+                                           " is not strictly increasing or has an id >= n");
+        HIPCHK(hipMemcpyAsync(e->h_counter, offsets + n_c, sizeof(uint64_t), hipMemcpyDeviceToHost, e->stream));
+        e->sync();
+        const uint64_t total = *e->h_counter, nw = (total + 63) / 64;
+        if (nw > words_cap)
+            throw Fail(LPA_ERR_RANGE, "the stream needs " + std::to_string(nw) + " words, words_cap is " +
+                                          std::to_string(words_cap));
+        if (nw)
+            hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, e->stream,
+                               (unsigned long long *)words, nw, 0ull);
+        hipLaunchKernelGGL(k_encode_write, dim3((unsigned)((n_c + 255) / 256)), dim3(256), 0, e->stream, row, col, vid,
+                           n_c, offsets, words);
+        HIPCHK(hipGetLastError());
+        e->sync();
+        *nbits = total;
     });
 }
 

@@ -58,6 +58,8 @@ EXPORTS = {
     "lpa_calibration_copy": (C.c_int, [_VP, C.c_uint64, C.c_int, _VP]),
     "lpa_graph_encode_csr": (C.c_int, [_VP, C.c_uint64, _VP, _VP, C.c_uint32]),
     "lpa_graph_stream": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP]),
+    "lpa_encode_csr_device": (C.c_int, [_VP, C.c_uint64, C.c_uint64, _VP, _VP, _VP, _VP, _VP, C.c_uint64,
+                                        C.POINTER(C.c_uint64)]),
 }
 
 
