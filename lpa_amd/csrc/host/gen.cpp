@@ -92,18 +92,34 @@ double pl_mean(double xmin, double alpha, uint32_t cap) {
     return s / S;
 }
 
+// Copying model (directed).  Frozen rules, every draw from a counter-based
+// stream of the seed:
+//  * out-degree d(v) = min(cap, floor(x_min (1-u)^(-1/(alpha-1)))), u ~ U[0,1)
+//    (stream 4, index v), x_min set by bisection so that the mean of this
+//    degree over u (stratified quadrature) is mean_deg;
+//  * base list of u: B(u)[j] = uniform id (stream 8, index u * 2^17 + j),
+//    j < d(u);
+//  * prototype p(v) = v + U{-window..window} (stream 5, index v), clamped to
+//    [0, n-1], and v + 1 (mod n) when that is v itself;
+//  * v copies c(v) = min(round(copy_frac * d(v)), d(p)) entries of B(p(v)) at
+//    distinct positions -- a partial Fisher-Yates shuffle of the positions
+//    0..d(p)-1 driven by stream 7, index v * 2^17 + j -- so copies never
+//    repeat a position;
+//  * the other d(v) - c(v) targets are B(v)[c(v) .. d(v)): uniform ids;
+//  * self loops and repeated targets are removed by the CSR build (rare:
+//    both need a collision among uniform ids).
 void gen_copying(uint64_t n, double mean_deg, double alpha, uint32_t cap, double copy_frac,
                  uint32_t window, uint64_t seed, lpah_csr &out) {
     if (n < 2) throw Error(E_ARG, "copying model needs n >= 2");
     if (cap >= (1u << 17)) throw Error(E_RANGE, "degree cap must be < 2^17");
+    if (copy_frac < 0 || copy_frac > 1) throw Error(E_ARG, "copy fraction must be in [0, 1]");
     double lo = 0.05, hi = 1000.0;   // bisection on xmin for the requested mean
     for (int it = 0; it < 60; it++) {
         double mid = 0.5 * (lo + hi);
         (pl_mean(mid, alpha, cap) < mean_deg ? lo : hi) = mid;
     }
     const double xmin = 0.5 * (lo + hi);
-    const uint64_t kdeg = stream_key(seed, 4), kproto = stream_key(seed, 5),
-                   kcopy = stream_key(seed, 6), kpick = stream_key(seed, 7),
+    const uint64_t kdeg = stream_key(seed, 4), kproto = stream_key(seed, 5), kpick = stream_key(seed, 7),
                    krand = stream_key(seed, 8);
     std::vector<uint32_t> deg(n);
 #pragma omp parallel for schedule(static)
@@ -111,22 +127,30 @@ void gen_copying(uint64_t n, double mean_deg, double alpha, uint32_t cap, double
     std::vector<uint64_t> off(n + 1, 0);
     for (uint64_t v = 0; v < n; v++) off[v + 1] = off[v] + deg[v];
     std::vector<uint32_t> s(off[n]), d(off[n]);
-    // the j-th "random" target of u; copies draw from the prototype's list
-    auto rand_target = [&](uint64_t u, uint64_t j) { return (uint32_t)(ctr_rand(krand, (u << 17) | j) % n); };
-#pragma omp parallel for schedule(dynamic, 1024)
-    for (uint64_t v = 0; v < n; v++) {
-        int64_t p = (int64_t)v + (int64_t)(ctr_rand(kproto, v) % (2ull * window + 1)) - (int64_t)window;
-        p = std::min<int64_t>(std::max<int64_t>(p, 0), (int64_t)n - 1);
-        if ((uint64_t)p == v) p = (int64_t)((v + 1) % n);
-        for (uint64_t j = 0; j < deg[v]; j++) {
-            uint64_t idx = (v << 17) | j;
-            uint32_t t;
-            if (deg[p] > 0 && u01(ctr_rand(kcopy, idx)) < copy_frac)
-                t = rand_target((uint64_t)p, ctr_rand(kpick, idx) % deg[p]);
-            else
-                t = rand_target(v, j);
-            s[off[v] + j] = (uint32_t)v;
-            d[off[v] + j] = t;
+    auto base = [&](uint64_t u, uint64_t j) { return (uint32_t)(ctr_rand(krand, (u << 17) | j) % n); };
+#pragma omp parallel
+    {
+        std::vector<uint32_t> pos;   // positions of the prototype's list not taken yet
+#pragma omp for schedule(dynamic, 1024)
+        for (uint64_t v = 0; v < n; v++) {
+            int64_t p = (int64_t)v + (int64_t)(ctr_rand(kproto, v) % (2ull * window + 1)) - (int64_t)window;
+            p = std::min<int64_t>(std::max<int64_t>(p, 0), (int64_t)n - 1);
+            if ((uint64_t)p == v) p = (int64_t)((v + 1) % n);
+            const uint64_t dv = deg[v], dp = deg[p];
+            const uint64_t c = std::min<uint64_t>((uint64_t)std::llround(copy_frac * (double)dv), dp);
+            pos.resize(dp);
+            for (uint64_t k = 0; k < dp; k++) pos[k] = (uint32_t)k;
+            uint32_t *sv = &s[off[v]], *tv = &d[off[v]];
+            for (uint64_t j = 0; j < c; j++) {
+                const uint64_t r = j + ctr_rand(kpick, (v << 17) | j) % (dp - j);
+                std::swap(pos[j], pos[r]);
+                sv[j] = (uint32_t)v;
+                tv[j] = base((uint64_t)p, pos[j]);
+            }
+            for (uint64_t j = c; j < dv; j++) {
+                sv[j] = (uint32_t)v;
+                tv[j] = base(v, j);
+            }
         }
     }
     build_csr(n, s, d, false, out);
