@@ -105,6 +105,7 @@ struct HeavySet {
     DevBuf<uint64_t> tab;
     DevBuf<uint32_t> sketch;
     DevBuf<unsigned long long> result;
+    DevBuf<uint32_t> hcnt;                   // CMS: own-label counters per (slot, vector of a batch)
     uint32_t ntasks = 0;
     DevBuf<DecodeTask> tasks;                // decode tasks (fixed per layout)
     DevBuf<uint32_t> stash;                  // CMS, several vectors: label stash per resident workgroup
@@ -398,6 +399,7 @@ void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
     h.tab = hs.tab.p;
     h.sketch = hs.sketch.p;
     h.result = hs.result.p;
+    h.hcnt = hs.hcnt.p;
     h.nchunks = hs.nchunks;
     h.chunk_h = hs.chunk_h.p;
     h.chunk_b = hs.chunk_b.p;
@@ -423,16 +425,12 @@ void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
         const uint32_t nsmall = hs.nlocal - hs.nlocal_big;
         const uint32_t *lsmall = hs.local.p, *lbig = hs.local.p + nsmall;
         if (hs.cms) {
-            if (one) {
-                hipLaunchKernelGGL((k_heavy_local<1, 1, 4>), dim3(hs.nlocal), b, sk, stream, h, hs.local.p, hs.nlocal);
-            } else {
-                // resident workgroups loop over the vertices, each with its stash
-                h.stash = hs.stash.p;
-                h.stash_stride = (uint64_t)CMS_LOCAL * HEAVY_KB_CMS_LOCAL;
-                const uint32_t g = hs.stash_blocks ? std::min(hs.nlocal, hs.stash_blocks) : hs.nlocal;
-                hipLaunchKernelGGL((k_heavy_local<1, HEAVY_KB_CMS_LOCAL, 4>), dim3(g), b, sk, stream, h, hs.local.p,
-                                   hs.nlocal);
-            }
+            // resident workgroups loop over the vertices, each with its stash
+            h.stash = hs.stash.p;
+            h.stash_stride = (uint64_t)CMS_LOCAL * (one ? 1 : HEAVY_KB_CMS_LOCAL);
+            const dim3 g(std::min(hs.nlocal, hs.stash_blocks));
+            if (one) hipLaunchKernelGGL((k_heavy_local<1, 1, 4>), g, b, sk, stream, h, hs.local.p, hs.nlocal);
+            else hipLaunchKernelGGL((k_heavy_local<1, HEAVY_KB_CMS_LOCAL, 4>), g, b, sk, stream, h, hs.local.p, hs.nlocal);
         } else {
             // tables of up to 2*CHUNK (8192 ids: 4 per lane) or 4*CHUNK slots (8 per lane)
             const size_t t1 = 2 * CHUNK * sizeof(uint64_t), t2 = 4 * CHUNK * sizeof(uint64_t);
@@ -1028,6 +1026,7 @@ static void build_heavy(lpa_engine *e) {
             if (hs->cms) {
                 hs->kb = e->K;
                 hs->sketch.alloc(((uint64_t)e->cms_depth * e->cms_width) * std::max<uint32_t>(hs->nchunked, 1) * hs->kb);
+                hs->hcnt.alloc(4ull * std::max<uint32_t>(hs->nchunked, 1) * hs->kb);
             } else {
                 const uint64_t per_vec = toff.back() * 8;
                 hs->kb = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(e->K, table_budget / std::max<uint64_t>(per_vec, 1)));
@@ -1040,15 +1039,18 @@ static void build_heavy(lpa_engine *e) {
                 HIPCHK(hipMemsetAsync(hs->epoch.p, 0, sizeof(uint32_t), e->stream));
             }
             hs->result.alloc((uint64_t)std::max<uint32_t>(hs->nchunked, 1) * hs->kb);
-            if (hs->cms && !hs->csr_part && hs->nlocal && e->K > 1) {
-                // one stash of CMS_LOCAL rows of HEAVY_KB_CMS_LOCAL labels per workgroup the CUs hold at once
+            if (hs->cms && hs->nlocal) {
+                // one compacted stash (CMS_LOCAL entries per label vector of a
+                // batch) per workgroup the CUs hold at once
                 const size_t cells = (size_t)e->cms_depth * e->cms_width;
+                const bool one = e->K == 1;
                 const uint32_t kbl = cms_local_kb(e->K, cells, lds_budget(e));
+                const void *fn = one ? (const void *)k_heavy_local<1, 1, 4>
+                                     : (const void *)k_heavy_local<1, HEAVY_KB_CMS_LOCAL, 4>;
                 int per_cu = 0;
-                HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    &per_cu, (const void *)k_heavy_local<1, HEAVY_KB_CMS_LOCAL, 4>, HBLOCK, cms_local_lds(kbl, cells)));
+                HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, HBLOCK, cms_local_lds(kbl, cells)));
                 hs->stash_blocks = (uint32_t)std::min<uint64_t>(hs->nlocal, (uint64_t)std::max(per_cu, 1) * e->num_cu);
-                hs->stash.alloc((uint64_t)hs->stash_blocks * CMS_LOCAL * HEAVY_KB_CMS_LOCAL);
+                hs->stash.alloc((uint64_t)hs->stash_blocks * CMS_LOCAL * (one ? 1 : HEAVY_KB_CMS_LOCAL));
             }
             e->heavy.push_back(std::move(hs));
         }

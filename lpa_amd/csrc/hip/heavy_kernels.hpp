@@ -62,6 +62,8 @@ struct HeavyArgs {
     uint32_t *epoch;                           // exact mode, tagged tables: passes completed (device word)
     int tagged;                                // exact mode: HBM table slots carry the pass's generation
     uint32_t kb_alloc;                         // exact mode: vectors the table buffer is laid out for
+    uint32_t *hcnt;                            // CMS, chunked hubs: per (slot, vector of the batch) 4 words:
+                                               // own-label count, labelled count, full-collision flag, unused
 };
 
 // Exact-mode HBM tables with generation tags.  A slot's count word holds
@@ -222,6 +224,7 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_clear(HeavyArgs h, const uint32
         const uint64_t cells = ((uint64_t)h.s.cms_depth << h.s.cms_logw) * h.kb;
         uint32_t *sk = h.sketch + (uint64_t)sl * cells;
         for (uint64_t c = threadIdx.x; c < cells; c += BLOCK) sk[c] = 0;
+        for (uint32_t k = threadIdx.x; k < 4 * h.kb; k += BLOCK) h.hcnt[(uint64_t)sl * h.kb * 4 + k] = 0;
     } else if (!h.tagged) {
         for (uint64_t c = h.toff[i] * h.kb + threadIdx.x; c < h.toff[i + 1] * h.kb; c += BLOCK) h.tab[c] = EMPTY_SLOT;
     } else if (*h.epoch % TABLE_GENS == 0) {
@@ -274,7 +277,7 @@ constexpr int HEAVY_KB_CMS_LOCAL = 8;
 template <bool ADD, int KB, bool PACK = false>
 __device__ __forceinline__ void cms_rounds(const StepArgs &a, const uint32_t *__restrict__ ids, uint32_t n,
                                            uint32_t k0, uint32_t kb, uint32_t *sk, uint32_t cells,
-                                           uint64_t (&best)[KB], uint32_t *stash = nullptr) {
+                                           uint64_t (&best)[KB], uint32_t *stash = nullptr, uint32_t qmask = ~0u) {
     // neighbours per lane and round: fewer when a row carries 8 labels
     constexpr int R = KB > 4 ? 1 : CMS_R;
     // the row hashes stay in scalar registers for the whole pass (a runtime
@@ -337,7 +340,7 @@ __device__ __forceinline__ void cms_rounds(const StepArgs &a, const uint32_t *__
         }
 #pragma unroll
         for (int q = 0; q < KB; q++) {
-            if ((uint32_t)q >= kb) continue;
+            if ((uint32_t)q >= kb || !((qmask >> q) & 1)) continue;
             uint32_t *skq = sk + (uint64_t)(PACK ? q >> 1 : q) * cells;
             const uint32_t half = PACK ? 16u * (q & 1) : 0u;   // a constant once unrolled
 #pragma unroll
@@ -369,6 +372,165 @@ __device__ __forceinline__ void cms_rounds(const StepArgs &a, const uint32_t *__
     }
 }
 
+// LDS a one-workgroup CMS vertex (or a chunk) needs beside its sketches: per
+// vector the compacted stash cursor, the collision bit and the cross-wave
+// sums of the packed own-label / labelled counts.
+struct CmsLocalShared {
+    uint32_t cur[HEAVY_KB_CMS_LOCAL];
+    uint32_t coll;
+    uint32_t xw[HEAVY_KB_CMS_LOCAL * (HBLOCK / 64)], xs[HEAVY_KB_CMS_LOCAL];   // block_sum_words
+};
+
+// The sketch's row hashes h_r(x) = (a_r x + b_r mod 2^64) >> (64 - log2 w),
+// held in registers for a whole pass (indexing the kernel-argument block with
+// a runtime row would reload them per label).
+struct CmsRows {
+    static constexpr int MAXD = 8;
+    uint64_t ha[MAXD], hb[MAXD];
+    uint32_t depth, logw;
+    __device__ __forceinline__ explicit CmsRows(const StepArgs &a) : depth(a.cms_depth), logw(a.cms_logw) {
+#pragma unroll
+        for (int r = 0; r < MAXD; r++) {
+            ha[r] = a.cms_ab[2 * r];
+            hb[r] = a.cms_ab[2 * r + 1];
+        }
+    }
+    __device__ __forceinline__ uint32_t col(int r, uint32_t lab) const {
+        return logw ? (uint32_t)((ha[r] * (uint64_t)lab + hb[r]) >> (64 - logw)) : 0u;
+    }
+    // estimate of lab in sketch q (16-bit half q & 1 of the packed words of
+    // pair q >> 1 with PACK, else the 32-bit cells of sketch q)
+    template <bool PACK>
+    __device__ __forceinline__ uint32_t estimate(const uint32_t *sk, uint32_t cells, uint32_t q, uint32_t lab) const {
+        const uint32_t *skq = sk + (uint64_t)(PACK ? q >> 1 : q) * cells;
+        const uint32_t half = PACK ? 16u * (q & 1) : 0u;
+        uint32_t est = UNL;
+#pragma unroll
+        for (int r = 0; r < MAXD; r++) {
+            if ((uint32_t)r >= depth) break;
+            const uint32_t w = skq[((uint32_t)r << logw) + col(r, lab)];
+            const uint32_t x = PACK ? (w >> half) & 0xFFFFu : w;
+            est = x < est ? x : est;
+        }
+        return est;
+    }
+};
+
+// Own labels of vectors [k0, k0+kb) of vertex v (clean; UNL beyond kb) and
+// their row-0 columns.
+template <int KB>
+__device__ __forceinline__ void cms_own(const StepArgs &a, const CmsRows &H, uint32_t v, uint32_t k0, uint32_t kb,
+                                        uint32_t (&own)[KB], uint32_t (&own0)[KB]) {
+#pragma unroll
+    for (int q = 0; q < KB; q++) {
+        own[q] = (uint32_t)q < kb ? clean_label(a.lin[(uint64_t)v * a.K + k0 + q]) : UNL;
+        own0[q] = H.col(0, own[q]);
+    }
+}
+
+// Pass 1 of a count-min step over ids[0..n) (n < 2^16) for vectors [k0,
+// k0+kb) (LP-SC rule 4), sketches sk in LDS (PACK: 16-bit cells, two vectors
+// to a word; else 32-bit cells).  Every neighbour's label row is gathered
+// once.  Per vector, copies of the vertex's own label are only counted (pc,
+// packed own / labelled counts of the wave); every other labelled entry is
+// added to the sketch (a wave's copies of one label folded into one update)
+// and checked for sharing the own label's cell in every row (cs->coll bit
+// q).  stash (or null): one entry per folded group is appended to
+// stash + q * cap (cursor cs->cur[q]), so the stash holds every distinct
+// non-own label present.  inplace (or null, one vector only): every
+// gathered label is written over its id.
+template <bool PACK, int KB>
+__device__ __forceinline__ void cms_add_pass(const StepArgs &a, const CmsRows &H, const uint32_t *ids, uint32_t n,
+                                             uint32_t k0, uint32_t kb, const uint32_t (&own)[KB],
+                                             const uint32_t (&own0)[KB], uint32_t *sk, uint32_t cells,
+                                             CmsLocalShared *cs, uint32_t *stash, uint32_t cap, uint32_t *inplace,
+                                             uint32_t (&pc)[KB]) {
+    const uint32_t lane = __lane_id();
+    const uint64_t lt = (1ull << lane) - 1;   // lanes below this one
+#pragma unroll
+    for (int q = 0; q < KB; q++) pc[q] = 0;
+    for (uint32_t i0 = 0; i0 < n; i0 += HBLOCK) {
+        const uint32_t i = i0 + threadIdx.x;
+        uint32_t labs[KB];
+        if (i < n) {
+            load_row<KB>(a, ids[i], k0, kb, labs);
+            if (KB == 1 && inplace) inplace[i] = labs[0];
+        } else {
+#pragma unroll
+            for (int q = 0; q < KB; q++) labs[q] = UNL;
+        }
+#pragma unroll
+        for (int q = 0; q < KB; q++) {
+            if ((uint32_t)q >= kb) continue;
+            const uint32_t lab = labs[q];
+            const bool is_own = own[q] != UNL && lab == own[q];
+            pc[q] += ((uint32_t)__popcll(__ballot(is_own)) << 16) + (uint32_t)__popcll(__ballot(lab != UNL));
+            if (lab == UNL || is_own) continue;
+            const uint32_t cnt = fold_first_label<HBLOCK>(lab);
+            if (cnt == 0) continue;
+            uint32_t *skq = sk + (uint64_t)(PACK ? q >> 1 : q) * cells;
+            const uint32_t half = PACK ? 16u * (q & 1) : 0u;
+            // full: lab shares the own label's cell in every row so far
+            bool full = own[q] != UNL;
+#pragma unroll
+            for (int r = 0; r < CmsRows::MAXD; r++) {
+                if ((uint32_t)r >= H.depth) break;
+                const uint32_t c = H.col(r, lab);
+                atomicAdd(&skq[((uint32_t)r << H.logw) + c], cnt << half);
+                if (full) full = c == (r == 0 ? own0[q] : H.col(r, own[q]));
+            }
+            if (full) atomicOr(&cs->coll, 1u << q);
+            if (stash) {
+                // one stash entry per folded group, compacted per wave
+                const uint64_t m = __ballot(1);
+                const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(&cs->cur[q], (uint32_t)__popcll(m));
+                base = (uint32_t)__shfl((int)base, (int)leader, 64);
+                stash[(uint64_t)q * cap + base + (uint32_t)__popcll(m & lt)] = lab;
+            }
+        }
+    }
+}
+
+// Workgroup totals of the packed counts (every thread gets them), then the
+// own label's copies enter the sketch as one update per vector.
+template <bool PACK, int KB>
+__device__ __forceinline__ void cms_own_totals(const CmsRows &H, uint32_t kb, const uint32_t (&own)[KB],
+                                               uint32_t *sk, uint32_t cells, CmsLocalShared *cs, uint32_t (&pc)[KB]) {
+    block_sum_words<KB>(pc, cs->xw, cs->xs);
+    if (threadIdx.x < kb) {
+#pragma unroll
+        for (int q = 0; q < KB; q++) {
+            const uint32_t oc = own_count(pc[q]);
+            if ((uint32_t)q != threadIdx.x || own[q] == UNL || oc == 0) continue;
+            uint32_t *skq = sk + (uint64_t)(PACK ? q >> 1 : q) * cells;
+            const uint32_t half = PACK ? 16u * (q & 1) : 0u;
+#pragma unroll
+            for (int r = 0; r < CmsRows::MAXD; r++) {
+                if ((uint32_t)r >= H.depth) break;
+                atomicAdd(&skq[((uint32_t)r << H.logw) + H.col(r, own[q])], oc << half);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// A vector is settled by its own label when that label is a strict majority
+// of the labelled neighbours and no present label shares its cell in every
+// row: any other label's estimate is then bounded by a cell holding none of
+// the own label's copies, so by fewer than half of the labelled neighbours,
+// while the own label's estimate is at least its count.
+__device__ __forceinline__ bool cms_settled(uint32_t own, uint32_t packed, bool coll) {
+    return own_majority_of(own, packed) && !coll;
+}
+
+// The same test from a chunked hub's counters (own count, labelled count,
+// full-collision flag; 32-bit words, a hub's list may exceed 2^16).
+__device__ __forceinline__ bool hub_settled(uint32_t own, const uint32_t *hc) {
+    return own != UNL && 2ull * hc[0] > hc[1] && hc[2] == 0;
+}
+
 // PASS 1: sketch increments, 2: sketch queries, 0: exact counting, for the
 // label vectors [k0, k0+kb) of one chunk.  Passes 1 and 0 first aggregate
 // the chunk in LDS (chunk-local sketches, or a chunk-local hash table of
@@ -380,6 +542,7 @@ template <int PASS, int KB>
 __global__ void __launch_bounds__(HBLOCK) k_heavy_chunks(HeavyArgs h) {
     extern __shared__ uint64_t lds[];
     __shared__ uint64_t red[HBLOCK / 64];
+    __shared__ CmsLocalShared cs[1];
     const uint32_t c = blockIdx.x;
     if (c >= h.nchunks) return;
     const uint32_t i = h.chunk_h[c];
@@ -390,6 +553,7 @@ __global__ void __launch_bounds__(HBLOCK) k_heavy_chunks(HeavyArgs h) {
     const StepArgs &a = h.s;
     const uint64_t t0 = a.rec ? cycles() : 0;
     const uint32_t sl = h.slot[i];
+    const uint32_t v = h.csr ? a.cvid[h.pos[i]] : a.vid[h.pos[i]];
     if (PASS == 0) {
         const uint32_t gen = h.tagged ? table_gen(h) : 0;
         const uint64_t T = h.toff[i + 1] - h.toff[i];
@@ -399,8 +563,12 @@ __global__ void __launch_bounds__(HBLOCK) k_heavy_chunks(HeavyArgs h) {
         for (uint32_t kq = 0; kq < h.kb; kq += KB) {
             const uint32_t kb = h.kb - kq < (uint32_t)KB ? h.kb - kq : (uint32_t)KB;
             uint64_t best = 0;
+            // a chunk holds part of the hub's list: no own-label shortcut
+            uint32_t no_own[KB];
+#pragma unroll
+            for (int s = 0; s < KB; s++) no_own[s] = UNL;
             count_staged<HBLOCK, HEAVY_IPL, KB>(
-                a, ids, n, threadIdx.x, h.k0 + kq, kb,
+                a, ids, n, threadIdx.x, h.k0 + kq, kb, no_own, false, nullptr, nullptr,
                 [&](int) {
                     for (uint32_t x = threadIdx.x; x < LT; x += HBLOCK) ltab[x] = EMPTY_SLOT;
                     best = 0;
@@ -426,7 +594,8 @@ __global__ void __launch_bounds__(HBLOCK) k_heavy_chunks(HeavyArgs h) {
                     best = block_max(best, red);
                     if (threadIdx.x == 0 && best)
                         atomicMax(&h.result[(uint64_t)sl * h.kb + kq + s], (unsigned long long)best);
-                });
+                },
+                [&](int, uint64_t) {});
         }
     } else {
         // several vectors: the chunk-local sketches take 16-bit cells, two
@@ -439,13 +608,20 @@ __global__ void __launch_bounds__(HBLOCK) k_heavy_chunks(HeavyArgs h) {
             uint32_t *hsk = h.sketch + ((uint64_t)sl * h.kb + kq) * cells;
             uint64_t best[KB] = {};
             if (PASS == 1) {
+                // own-label counts go to the hub's counters; the other
+                // labels, and the chunk's own copies as one update, to the
+                // chunk-local sketches, merged into the hub's
                 uint32_t *lsk = (uint32_t *)lds;
                 const uint32_t words = (PACK ? (kb + 1) / 2 : kb) * cells;
                 for (uint32_t x = threadIdx.x; x < words; x += HBLOCK) lsk[x] = 0;
+                if (threadIdx.x == 0) cs->coll = 0;
                 __syncthreads();
-                cms_rounds<true, KB, PACK>(a, ids, n, h.k0 + kq, kb, lsk, cells, best,
-                                           KB == 1 && a.K == 1 && !h.csr ? const_cast<uint32_t *>(ids) : nullptr);
-                __syncthreads();
+                const CmsRows H(a);
+                uint32_t own[KB], own0[KB], pc[KB];
+                cms_own<KB>(a, H, v, h.k0 + kq, kb, own, own0);
+                cms_add_pass<PACK, KB>(a, H, ids, n, h.k0 + kq, kb, own, own0, lsk, cells, cs, nullptr, 0,
+                                       KB == 1 && a.K == 1 && !h.csr ? const_cast<uint32_t *>(ids) : nullptr, pc);
+                cms_own_totals<PACK, KB>(H, kb, own, lsk, cells, cs, pc);
                 for (uint32_t x = threadIdx.x; x < words; x += HBLOCK) {
                     const uint32_t w = lsk[x];
                     if (!w) continue;
@@ -457,13 +633,32 @@ __global__ void __launch_bounds__(HBLOCK) k_heavy_chunks(HeavyArgs h) {
                         atomicAdd(&hsk[x], w);
                     }
                 }
+                const uint32_t coll = cs->coll;
+#pragma unroll
+                for (int q = 0; q < KB; q++) {
+                    if ((uint32_t)q >= kb || threadIdx.x != (uint32_t)q) continue;
+                    uint32_t *hc = h.hcnt + ((uint64_t)sl * h.kb + kq + q) * 4;
+                    if (own_count(pc[q])) atomicAdd(&hc[0], own_count(pc[q]));
+                    if (lab_count(pc[q])) atomicAdd(&hc[1], lab_count(pc[q]));
+                    if ((coll >> q) & 1) atomicOr(&hc[2], 1u);
+                }
                 __syncthreads();
             } else {
-                cms_rounds<false, KB>(a, ids, n, h.k0 + kq, kb, hsk, cells, best,
-                                      KB == 1 && a.K == 1 && !h.csr ? const_cast<uint32_t *>(ids) : nullptr);
+                // vectors settled by the hub's own label need no estimates
+                uint32_t undecided = 0;
 #pragma unroll
                 for (int q = 0; q < KB; q++) {
                     if ((uint32_t)q >= kb) continue;
+                    const uint32_t *hc = h.hcnt + ((uint64_t)sl * h.kb + kq + q) * 4;
+                    const uint32_t own = clean_label(a.lin[(uint64_t)v * a.K + h.k0 + kq + q]);
+                    if (!hub_settled(own, hc)) undecided |= 1u << q;
+                }
+                if (!undecided) continue;
+                cms_rounds<false, KB>(a, ids, n, h.k0 + kq, kb, hsk, cells, best,
+                                      KB == 1 && a.K == 1 && !h.csr ? const_cast<uint32_t *>(ids) : nullptr, undecided);
+#pragma unroll
+                for (int q = 0; q < KB; q++) {
+                    if ((uint32_t)q >= kb || !((undecided >> q) & 1)) continue;
                     const uint64_t m = block_max(best[q], red);
                     if (threadIdx.x == 0 && m)
                         atomicMax(&h.result[(uint64_t)sl * h.kb + kq + q], (unsigned long long)m);
@@ -477,9 +672,54 @@ __global__ void __launch_bounds__(HBLOCK) k_heavy_chunks(HeavyArgs h) {
     }
 }
 
+// One CMS vertex (neighbours ids[0..d)), label vectors [k0, k0+kb), by one
+// workgroup with kb sketches in LDS: pass 1 (cms_add_pass with the
+// compacted stash); vectors settled by their own label take it; for the
+// others, pass 2 estimates the own label and the stashed entries (every
+// distinct label present) and keeps the largest estimate, ties to the
+// smallest label.  result(q, key) is called by thread 0 with the vector's
+// (count << 32 | ~label).
+template <bool PACK, int KB, class Result>
+__device__ __forceinline__ void cms_local_own(const StepArgs &a, const uint32_t *__restrict__ ids, uint32_t d,
+                                              uint32_t v, uint32_t k0, uint32_t kb, uint32_t *sk, uint32_t cells,
+                                              uint32_t *stash, uint32_t cap, CmsLocalShared *cs, uint64_t *red,
+                                              Result &&result) {
+    const CmsRows H(a);
+    uint32_t own[KB], own0[KB], pc[KB];
+    cms_own<KB>(a, H, v, k0, kb, own, own0);
+    const uint32_t words = (PACK ? (kb + 1) / 2 : kb) * cells;
+    for (uint32_t x = threadIdx.x; x < words; x += HBLOCK) sk[x] = 0;
+    if (threadIdx.x < (uint32_t)KB) cs->cur[threadIdx.x] = 0;
+    if (threadIdx.x == 0) cs->coll = 0;
+    __syncthreads();
+    cms_add_pass<PACK, KB>(a, H, ids, d, k0, kb, own, own0, sk, cells, cs, stash, cap, nullptr, pc);
+    cms_own_totals<PACK, KB>(H, kb, own, sk, cells, cs, pc);
+    const uint32_t coll = cs->coll;
+#pragma unroll
+    for (int q = 0; q < KB; q++) {
+        if ((uint32_t)q >= kb) continue;
+        if (cms_settled(own[q], pc[q], (coll >> q) & 1)) {
+            if (threadIdx.x == 0) result(q, ((uint64_t)own_count(pc[q]) << 32) | (uint32_t)~own[q]);
+            continue;
+        }
+        uint64_t best = 0;
+        if (threadIdx.x == 0 && own[q] != UNL && own_count(pc[q]))
+            best = ((uint64_t)H.estimate<PACK>(sk, cells, q, own[q]) << 32) | (uint32_t)~own[q];
+        const uint32_t n = cs->cur[q];
+        for (uint32_t j = threadIdx.x; j < n; j += HBLOCK) {
+            const uint32_t lab = stash[(uint64_t)q * cap + j];
+            const uint64_t key = ((uint64_t)H.estimate<PACK>(sk, cells, q, lab) << 32) | (uint32_t)~lab;
+            best = key > best ? key : best;
+        }
+        best = block_max(best, red);
+        if (threadIdx.x == 0) result(q, best);
+    }
+    __syncthreads();   // the sketches, cursors and counts are reused next
+}
+
 template <int CMS, int KB, int IPL>
 __device__ __forceinline__ void heavy_local_vertex(const HeavyArgs &h, uint32_t i, uint64_t *lds, uint64_t *red,
-                                                   uint32_t *olds) {
+                                                   uint32_t *olds, uint32_t *xw, uint32_t *xs, CmsLocalShared *cs) {
     static_assert(KB <= HEAVY_KB_MAX, "olds[] holds one batch");
     const uint32_t d = heavy_deg(h, i);
     const uint32_t *ids = heavy_ids(h, i);
@@ -505,28 +745,12 @@ __device__ __forceinline__ void heavy_local_vertex(const HeavyArgs &h, uint32_t 
         // several vectors: 16-bit cells, two vectors to a word
         constexpr bool PACK = KB > 1;
         const uint32_t cells = a.cms_depth << a.cms_logw;
-        uint32_t *sk = (uint32_t *)lds;
+        uint32_t *stash = h.stash + (uint64_t)blockIdx.x * h.stash_stride;
         for (uint32_t k0 = 0; k0 < a.K; k0 += h.kbl) {
             const uint32_t kb = a.K - k0 < h.kbl ? a.K - k0 : h.kbl;
             prefetch(k0, kb);
-            const uint32_t words = (PACK ? (kb + 1) / 2 : kb) * cells;
-            for (uint32_t x = threadIdx.x; x < words; x += HBLOCK) sk[x] = 0;
-            __syncthreads();
-            uint64_t best[KB] = {};
-            // stash: the decoded list itself for one vector, else this
-            // workgroup's region of the stash buffer
-            uint32_t *stash = h.csr ? nullptr
-                            : KB == 1 && a.K == 1 ? const_cast<uint32_t *>(ids)
-                            : h.stash ? h.stash + (uint64_t)blockIdx.x * h.stash_stride : nullptr;
-            cms_rounds<true, KB, PACK>(a, ids, d, k0, kb, sk, cells, best, stash);
-            __syncthreads();
-            cms_rounds<false, KB, PACK>(a, ids, d, k0, kb, sk, cells, best, stash);
-#pragma unroll
-            for (int q = 0; q < KB; q++) {
-                if ((uint32_t)q >= kb) continue;
-                const uint64_t m = block_max(best[q], red);
-                if (threadIdx.x == 0) finish(k0, q, m);
-            }
+            cms_local_own<PACK, KB>(a, ids, d, v, k0, kb, (uint32_t *)lds, cells, stash, CMS_LOCAL, cs, red,
+                                    [&](uint32_t q, uint64_t key) { finish(k0, q, key); });
         }
     } else {
         const uint32_t T = d ? 1u << (32 - __builtin_clz(2 * d - 1)) : 1u;
@@ -536,8 +760,12 @@ __device__ __forceinline__ void heavy_local_vertex(const HeavyArgs &h, uint32_t 
             const uint32_t kb = a.K - k0 < (uint32_t)KB ? a.K - k0 : (uint32_t)KB;
             uint64_t best = 0;
             prefetch(k0, kb);
+            uint32_t own[KB];
+#pragma unroll
+            for (int s = 0; s < KB; s++)
+                own[s] = (uint32_t)s < kb ? clean_label(a.lin[(uint64_t)v * a.K + k0 + s]) : UNL;
             count_staged<HBLOCK, IPL, KB>(
-                a, ids, d, threadIdx.x, k0, kb,
+                a, ids, d, threadIdx.x, k0, kb, own, true, xw, xs,
                 [&](int) {
                     for (uint32_t x = threadIdx.x; x < T; x += HBLOCK) tab[x] = EMPTY_SLOT;
                     best = 0;
@@ -558,6 +786,9 @@ __device__ __forceinline__ void heavy_local_vertex(const HeavyArgs &h, uint32_t 
                 [&](int s) {
                     const uint64_t m = block_max(best, red);
                     if (threadIdx.x == 0) finish(k0, s, m);
+                },
+                [&](int s, uint64_t key) {
+                    if (threadIdx.x == 0) finish(k0, s, key);
                 });
         }
     }
@@ -576,18 +807,19 @@ __device__ __forceinline__ void heavy_local_vertex(const HeavyArgs &h, uint32_t 
 // one workgroup per CU, so those instances take four waves per SIMD and the
 // registers that frees.)
 template <int CMS, int KB, int IPL>
-__global__ void __launch_bounds__(HBLOCK, IPL == 8 ? 4 : 8) k_heavy_local(HeavyArgs h, const uint32_t *__restrict__ local,
-                                                       uint32_t nlocal) {
+__global__ void __launch_bounds__(HBLOCK, IPL == 8 || (CMS && KB > 1) ? 4 : 8)
+    k_heavy_local(HeavyArgs h, const uint32_t *__restrict__ local, uint32_t nlocal) {
     extern __shared__ uint64_t lds[];
     __shared__ uint64_t red[HBLOCK / 64];
     __shared__ uint32_t olds[HEAVY_KB_MAX];
+    __shared__ uint32_t xw[CMS ? 1 : HEAVY_KB_MAX * (HBLOCK / 64)], xs[HEAVY_KB_MAX];   // block_sum_words
+    __shared__ CmsLocalShared cs[1];
     if (CMS) {
-        // workgroups loop over the vertices (the grid may be smaller than
-        // the list when each workgroup owns a stash region)
+        // workgroups loop over the vertices, each with its stash region
         for (uint32_t j = blockIdx.x; j < nlocal; j += gridDim.x)
-            heavy_local_vertex<CMS, KB, IPL>(h, local[j], lds, red, olds);
+            heavy_local_vertex<CMS, KB, IPL>(h, local[j], lds, red, olds, xw, xs, cs);
     } else if (blockIdx.x < nlocal) {
-        heavy_local_vertex<CMS, KB, IPL>(h, local[blockIdx.x], lds, red, olds);
+        heavy_local_vertex<CMS, KB, IPL>(h, local[blockIdx.x], lds, red, olds, xw, xs, cs);
     }
 }
 
@@ -603,7 +835,13 @@ __global__ void __launch_bounds__(BLOCK) k_heavy_finish(HeavyArgs h, const uint3
         const uint32_t v = h.csr ? h.s.cvid[p] : h.s.vid[p];
         const uint64_t idx = (uint64_t)v * h.s.K + k;
         const uint32_t old = h.s.lin[idx];
-        const uint32_t nl = pick(h.result[t], old);
+        uint64_t key = h.result[t];
+        if (CMS) {
+            // settled by the hub's own label (the chunk passes skipped it)
+            const uint32_t *hc = h.hcnt + t * 4, own = clean_label(old);
+            if (hub_settled(own, hc)) key = ((uint64_t)hc[0] << 32) | (uint32_t)~own;
+        }
+        const uint32_t nl = pick(key, old);
         h.s.lout[idx] = nl;
         ch = nl != old;
     }

@@ -205,6 +205,8 @@ __global__ void __launch_bounds__(BLOCK) k_lane(StepArgs a) {
         const bool active = f < pairs;
         const uint32_t vb = active ? f / K : 0, k = active ? f - vb * K : 0;
         const uint32_t d = active ? sd[vb] : 0;
+        const uint64_t idx = (uint64_t)sv[vb] * K + k;
+        const uint32_t old = active ? a.lin[idx] : UNL, own = clean_label(old);
         uint32_t x[N];
 #pragma unroll
         for (int j = 0; j < N; j++) {
@@ -214,12 +216,23 @@ __global__ void __launch_bounds__(BLOCK) k_lane(StepArgs a) {
                     : CSR ? a.nbr[sbase[CSR ? vb : 0] + j] : list[CSR ? 0 : j][vb];
             x[j] = (uint32_t)j < d ? clean_label(a.lin[(uint64_t)u * K + k]) : UNL;
         }
-        bitonic_sort<N>(x);
-        const uint64_t best = longest_run<N>(x);
+        // own label a strict majority of the labelled neighbours -> it is
+        // the mode; otherwise sort and take the longest run
+        uint32_t oc = 0, lc = 0;
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            oc += x[j] == own && own != UNL;
+            lc += x[j] != UNL;
+        }
+        uint64_t best;
+        if (own != UNL && 2 * oc > lc) {
+            best = ((uint64_t)oc << 32) | (uint32_t)~own;
+        } else {
+            bitonic_sort<N>(x);
+            best = longest_run<N>(x);
+        }
         if (active) {
             const uint32_t v = sv[vb];
-            const uint64_t idx = (uint64_t)v * K + k;
-            const uint32_t old = a.lin[idx];
             const uint32_t nl = pick(best, old);
             a.lout[idx] = nl;
             nch += nl != old;
@@ -417,19 +430,84 @@ __device__ __forceinline__ void match_rounds(const uint32_t (&labs)[IPL][KB], in
     }
 }
 
+// Cross-wave sums of a workgroup (every thread calls it): lane 0 of each
+// wave puts its KB words in xw (KB * waves words), threads 0..KB-1 add them
+// up in xs (KB words), and every thread returns with the totals in x.  Two
+// barriers; xw and xs can be reused by the next call without another.
+template <int KB>
+__device__ __forceinline__ void block_sum_words(uint32_t (&x)[KB], uint32_t *xw, uint32_t *xs) {
+    const uint32_t w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int s = 0; s < KB; s++) xw[s * nw + w] = x[s];
+    }
+    __syncthreads();
+    if (threadIdx.x < (uint32_t)KB) {
+        uint32_t t = 0;
+        for (uint32_t ww = 0; ww < nw; ww++) t += xw[threadIdx.x * nw + ww];
+        xs[threadIdx.x] = t;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < KB; s++) x[s] = xs[s];
+}
+
+// Own-label count and labelled count of a vector, packed as (own << 16 |
+// labelled): every list the own-label check sees is shorter than 2^16.
+__device__ __forceinline__ uint32_t own_count(uint32_t packed) { return packed >> 16; }
+__device__ __forceinline__ uint32_t lab_count(uint32_t packed) { return packed & 0xFFFFu; }
+__device__ __forceinline__ bool own_majority_of(uint32_t own, uint32_t packed) {
+    return own != UNL && 2 * own_count(packed) > lab_count(packed);
+}
+
+// The vertex's own label decides a vector when it is a strict majority of
+// the labelled neighbours: its count then exceeds the sum of all the others,
+// so it is the most frequent label, with no tie (LP-SC exact mode).  In a
+// settled neighbourhood this holds for nearly every vertex and vector, and
+// the check costs a compare and a ballot per staged label instead of a
+// table.  own[s] is the clean own label of vector k0+s (UNL: never decides);
+// every lane of a group holds the same own[] and reaches this together.
+// xw, xs: LDS scratch of block_sum_words when G > 64 (the group's counts are
+// summed across its waves), unused otherwise.  Returns the mask of decided
+// vectors; pc[s] = the packed counts.
+template <int G, int IPL, int KB>
+__device__ __forceinline__ uint32_t own_majority(const uint32_t (&labs)[IPL][KB], const uint32_t (&own)[KB],
+                                                 uint32_t kb, uint32_t *xw, uint32_t *xs, uint32_t (&pc)[KB]) {
+    const uint64_t gm = group_mask<G>();
+#pragma unroll
+    for (int s = 0; s < KB; s++) {
+        pc[s] = 0;
+#pragma unroll
+        for (int t = 0; t < IPL; t++) {
+            pc[s] += (uint32_t)__popcll(__ballot(own[s] != UNL && labs[t][s] == own[s]) & gm) << 16;
+            pc[s] += (uint32_t)__popcll(__ballot(labs[t][s] != UNL) & gm);
+        }
+    }
+    if (G > 64) block_sum_words<KB>(pc, xw, xs);
+    uint32_t mask = 0;
+#pragma unroll
+    for (int s = 0; s < KB; s++)
+        if ((uint32_t)s < kb && own_majority_of(own[s], pc[s])) mask |= 1u << s;
+    return mask;
+}
+
 // Phase B: counts label vectors k0 .. k0+kb-1 (kb <= KB, the same in every
 // lane of the group) of the neighbours ids[0..d), one vector at a time.
 // Lane gl first gathers the label-row slices of neighbours gl, gl+G, ...,
 // gl+(IPL-1)G into registers (IPL*KB values, all loads in flight together),
 // so a vertex's neighbour rows are read once per batch of KB vectors;
-// neighbours beyond IPL*G are gathered per vector.  For each vector s:
-// begin(s) prepares the table (and synchronises the group); staged labels
-// go through aggregated rounds (agg(s, lab, count) by one lane per match)
-// and then use(s, lab) one at a time; end(s) finishes the vector.
-template <int G, int IPL, int KB, class Begin, class Agg, class Use, class End>
+// neighbours beyond IPL*G are gathered per vector.  With use_own (the list
+// is the vertex's whole neighbourhood and all of it is staged), a vector
+// whose own label is a strict majority is settled by decide(s, key) with
+// no table work; for every other vector s: begin(s) prepares the table (and
+// synchronises the group); staged labels go through aggregated rounds
+// (agg(s, lab, count) by one lane per match) and then use(s, lab) one at a
+// time; end(s) finishes the vector.
+template <int G, int IPL, int KB, class Begin, class Agg, class Use, class End, class Decide>
 __device__ __forceinline__ void count_staged(const StepArgs &a, const uint32_t *__restrict__ ids, uint32_t d,
-                                             uint32_t gl, uint32_t k0, uint32_t kb, Begin &&begin, Agg &&agg,
-                                             Use &&use, End &&end) {
+                                             uint32_t gl, uint32_t k0, uint32_t kb, const uint32_t (&own)[KB],
+                                             bool use_own, uint32_t *xw, uint32_t *xs, Begin &&begin, Agg &&agg,
+                                             Use &&use, End &&end, Decide &&decide) {
     uint32_t labs[IPL][KB];
 #pragma unroll
     for (int t = 0; t < IPL; t++) {
@@ -441,9 +519,15 @@ __device__ __forceinline__ void count_staged(const StepArgs &a, const uint32_t *
             for (int s = 0; s < KB; s++) labs[t][s] = UNL;
         }
     }
+    uint32_t decided = 0, pc[KB];
+    if (use_own && d <= (uint32_t)(IPL * G)) decided = own_majority<G, IPL, KB>(labs, own, kb, xw, xs, pc);
 #pragma unroll
     for (int s = 0; s < KB; s++) {
         if ((uint32_t)s < kb) {
+            if (decided & (1u << s)) {
+                decide(s, ((uint64_t)own_count(pc[s]) << 32) | (uint32_t)~own[s]);
+                continue;
+            }
             begin(s);
             uint32_t pm = 0;
 #pragma unroll
@@ -472,6 +556,7 @@ __global__ void __launch_bounds__(BLOCK) k_group(StepArgs a, uint32_t logT, uint
     constexpr int NG = BLOCK / G;
     extern __shared__ uint64_t lds[];
     __shared__ uint64_t red[BLOCK / 64];
+    __shared__ uint32_t xw[G > 64 ? KB * (G / 64) : 1], xs[KB];
     __shared__ GroupShared<G> gsh[NG];
     const uint32_t gid = threadIdx.x / G, gl = threadIdx.x % G;
     const uint32_t T = 1u << logT;
@@ -499,8 +584,12 @@ __global__ void __launch_bounds__(BLOCK) k_group(StepArgs a, uint32_t logT, uint
         for (uint32_t k0 = 0; k0 < a.K; k0 += KB) {
             const uint32_t kb = a.K - k0 < (uint32_t)KB ? a.K - k0 : (uint32_t)KB;
             uint64_t best = 0, mine = 0;   // lane gl < kb keeps vector k0+gl's result
+            uint32_t own[KB];
+#pragma unroll
+            for (int s = 0; s < KB; s++)
+                own[s] = live && (uint32_t)s < kb ? clean_label(a.lin[(uint64_t)v * a.K + k0 + s]) : UNL;
             count_staged<G, IPL, KB>(
-                a, ids, d, gl, k0, kb,
+                a, ids, d, gl, k0, kb, own, true, xw, xs,
                 [&](int) {
                     for (uint32_t i = gl; i < T; i += G) tab[i] = EMPTY_SLOT;
                     best = 0;
@@ -528,6 +617,9 @@ __global__ void __launch_bounds__(BLOCK) k_group(StepArgs a, uint32_t logT, uint
                     }
                     if (gl == (uint32_t)s) mine = best;
                     group_sync<G>();   // table / red reused next
+                },
+                [&](int s, uint64_t key) {
+                    if (gl == (uint32_t)s) mine = key;
                 });
             // the batch's labels are picked and written by kb lanes at once
             // (one memory round trip per batch, not per vector)

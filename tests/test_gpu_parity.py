@@ -133,6 +133,60 @@ def test_graph_replay_matches_eager():
         assert np.array_equal(eager.labels(), graph.labels()), it
 
 
+@pytest.mark.parametrize("K,cms", [(1, False), (9, False), (8, True)])
+def test_graph_replay_hubs_across_generation_wrap(K, cms):
+    """steps replayed from captured HIP graphs on the hub graph (the
+    one-workgroup and chunked heavy kernels: generation-tagged exact tables,
+    count-min sketches and the own-label counters) match eager steps, from
+    few-valued random labels, across the exact tables' generation wrap (the
+    pass counter reaches 255 after 250 warm-up steps)"""
+    cgr = _hub_graph().encode(0, 128)
+    cfg = dict(cms_thresh=2500, cms_depth=4, cms_width=1024, cms_seed=4) if cms else {}
+    print(f"[gpu-case] replay hubs K={K} cms={cms}", flush=True)
+    eager = make_engine(cgr, K=K, **cfg)
+    graph = make_engine(cgr, K=K, **cfg)
+    eager.labels_init()
+    graph.labels_init()
+    for _ in range(250):
+        eager.nova_iterate()
+        graph.nova_iterate()
+    rng = np.random.default_rng(K)
+    lab = rng.integers(0, 30, cgr.n * K).astype(np.uint32)
+    eager.labels_init(lab)
+    graph.labels_init(lab)
+    graph.set_graph_mode(True)
+    for it in range(8):
+        a = eager.nova_iterate()
+        b = graph.nova_iterate()
+        assert a == b and np.array_equal(eager.labels(), graph.labels()), it
+        graph.degrees()          # unrelated eager work between replays
+    eager.close()
+    graph.close()
+
+
+def test_graph_replay_semi_supervised_k64():
+    """64 semi-supervised vectors (1 % seeds from 100 classes, clamped),
+    exact mode: the batched-instance configuration, replayed"""
+    g = L.gen_rmat(16, 16, seed=33, bfs=True)
+    cgr = g.encode(0, 128)
+    n, K = g.n, 64
+    rng = np.random.default_rng(3)
+    lab = np.full((n, K), 0xFFFFFFFF, np.uint32)
+    for k in range(K):
+        idx = rng.choice(n, n // 100, replace=False)
+        lab[idx, k] = rng.integers(0, 100, len(idx), dtype=np.uint32)
+    eager = make_engine(cgr, K=K, semi=True)
+    graph = make_engine(cgr, K=K, semi=True)
+    eager.labels_init(lab.reshape(-1))
+    graph.labels_init(lab.reshape(-1))
+    graph.set_graph_mode(True)
+    for it in range(10):
+        a = eager.nova_iterate()
+        b = graph.nova_iterate()
+        assert a == b and np.array_equal(eager.labels(), graph.labels()), it
+    run_and_compare(cgr, 3, labels0=lab.reshape(-1), K=K, semi=True, engine=graph)
+
+
 def test_autotune_keeps_results():
     g = L.gen_rmat(14, 16, seed=32)
     cgr = g.encode(0, 128)
