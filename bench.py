@@ -50,8 +50,10 @@ WORKLOADS = {
     "copy50m": dict(gen="copying", n=50_000_000, seed=2, T=32, K=1,
                     cms=dict(cms_thresh=8192, cms_depth=4, cms_width=2048, cms_seed=0)),
     # R-MAT scale 24, ef 16, seed 3; K = 64 semi-supervised vectors (1% seeds
-    # from 100 classes, clamped); exact
-    "rmat24k64": dict(gen="rmat", scale=24, ef=16, seed=3, T=0, K=64, semi=True, cms=dict()),
+    # from 100 classes, clamped); exact; autotuned bins, steps replayed from
+    # a captured HIP graph
+    "rmat24k64": dict(gen="rmat", scale=24, ef=16, seed=3, T=0, K=64, semi=True, cms=dict(), graph=True,
+                      autotune=True),
     # R-MAT scale 27, ef 32, seed 4; K = 8; CMS 4x2048 for deg > 8192
     "rmat27": dict(gen="rmat", scale=27, ef=32, seed=4, T=0, K=8,
                    cms=dict(cms_thresh=8192, cms_depth=4, cms_width=2048, cms_seed=0)),
@@ -76,7 +78,9 @@ def log(msg):
 def build_graph(w):
     import lpa_amd as L
     if w["gen"] == "rmat":
-        g = L.gen_rmat(w["scale"], w["ef"], 0.57, 0.19, 0.19, seed=w["seed"], bfs=True)
+        # drawn on the host, BFS-relabelled on the device (the same graph as
+        # the host relabelling, bit for bit)
+        g = L.gen_rmat_device_bfs(w["scale"], w["ef"], 0.57, 0.19, 0.19, seed=w["seed"])
     elif w["gen"] == "copying":
         g = L.gen_copying(w["n"], 30.0, 2.1, 100000, 0.7, 1000, seed=w["seed"])
     else:
@@ -108,12 +112,17 @@ def alg_bytes(st, K):
         + 4.0 * st["csr_arcs"]
 
 
-def timed(e, steps, warmup, method="nova", labels=None):
+def timed(e, steps, warmup, method="nova", labels=None, use_graph=False):
+    """warmup untimed steps, then `steps` steps timed with HIP events (per
+    kernel family too when eager); use_graph replays captured steps"""
     e.labels_init(labels)
+    e.set_graph_mode(use_graph)
     for _ in range(warmup):
         {"nova": e.nova_iterate, "cglp": e.cglp_iterate, "csr": e.csr_iterate}[method]()
-    ms, changed = e.time_iterations(steps, method)
-    return ms, changed, e.kernel_times()
+    ms, changed = e.time_iterations(steps, method, use_graph=use_graph)
+    kt = e.kernel_times()
+    e.set_graph_mode(False)
+    return ms, changed, kt
 
 
 def train_model(L, cgr_main, w, K):
@@ -213,17 +222,31 @@ def main():
         kept, judge_l, judge_d = e.schedule_learned_judged(model, iters=5)
         log(f"judge: learned layout {judge_l:.3f} ms, degree layout {judge_d:.3f} ms -> "
             f"{'learned' if kept else 'degree'} kept")
-        msl, chl, ktl = timed(e, args.steps, args.warmup, labels=labK)
+        layout = "learned" if kept else "degree"
+        if w.get("autotune") and not kept:
+            # the degree layout stays: its bin thresholds from the autotuner
+            tuned, tuned_ms = e.autotune(iters=2)
+            res["autotune"] = dict(params=tuned, median_ms=tuned_ms)
+            layout = "degree (autotuned)"
+            log(f"autotuned degree schedule {tuned}: {tuned_ms:.3f} ms/step")
+        use_graph = bool(w.get("graph"))
+        msl, chl, ktl = timed(e, args.steps, args.warmup, labels=labK, use_graph=use_graph)
+        if use_graph:
+            # the per-family split comes from the same steps run eagerly
+            mse, _, ktl = timed(e, args.steps, args.warmup, labels=labK)
+            res["eager_ms_per_step"] = float(np.mean(mse))
+            log(f"same steps eager: {np.mean(mse):.3f} ms/step")
         res[f"learned_k{K}"] = dict(edges_per_s=K * m / (float(np.mean(msl)) / 1e3),
                                     ms_per_step=float(np.mean(msl)), model_loss=[float(losses[0]), float(losses[-1])],
-                                    bins=e.counts(), layout="learned" if kept else "degree")
+                                    bins=e.counts(), layout=layout, use_graph=use_graph)
         res["learned_vs_degree"] = {
             "K": K, "learned_layout_ms": judge_l, "degree_layout_ms": judge_d,
             "learned_gain": judge_d / judge_l, "kept": "learned" if kept else "degree",
             "note": "median of 5 steps per layout from the same labels, measured degree, learned, degree"}
-        log(f"headline K={K} ({'learned' if kept else 'degree'} layout): {np.mean(msl):.3f} ms/step")
+        log(f"headline K={K} ({layout} layout{', HIP-graph replay' if use_graph else ''}): "
+            f"{np.mean(msl):.3f} ms/step")
         head_ms, head_kt = msl, ktl
-        schedule = "learned" if kept else "degree (learned layout rejected by the judge)"
+        schedule = "learned" if kept else layout + ", the learned layout rejected by the judge"
     if args.copy_calibration:
         cms_ = e.calibration_copy(256 << 20, 3)
         log(f"copy calibration: 3 copies of 256 MiB, {cms_:.3f} ms each ({2 * 256 * 1.048576 / cms_:.0f} GB/s)")
@@ -249,7 +272,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "n": int(g.n), "m": m, "cgr_bits": int(cgr.nbits), "K": K,
                    "semi_supervised": semi, "threshold": w["T"], "method": "nova-fused", "schedule": schedule,
-                   **w["cms"]},
+                   "hip_graph_replay": bool(w.get("graph")) and not args.no_learned, **w["cms"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
                      "kernel_ms_per_step": head_kt[dom], "alg_bytes_per_step": dom_bytes},

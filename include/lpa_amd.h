@@ -89,6 +89,10 @@ typedef struct {
     uint32_t block_max;     /* deg <= block_max -> BLOCK, else the heavy-vertex path (0: LDS limit) */
     uint32_t relayout;      /* records are always reordered so each bin is contiguous; keep 1 */
     uint32_t quarter_max;   /* WAVE vertices up to this degree use 16-lane groups (0: 128) */
+    uint32_t bin_order;     /* vertices inside a bin: 1 by vertex id (neighbouring ids run together),
+                               2 by degree, descending; 0: the engine's default (2) */
+    uint32_t xcd_tiles;     /* 1: the workgroups one XCD runs take consecutive tiles of a bin (each XCD
+                               has its own L2), 2: tiles in workgroup order; 0: the engine's default (2) */
 } lpa_sched_params;
 
 const char *lpa_last_error(void);
@@ -120,6 +124,16 @@ int lpa_graph_upload(lpa_engine *e, uint64_t n, uint64_t n_c,
  * duplicate or out-of-range ids (LPA_ERR_FORMAT). */
 int lpa_graph_encode_csr(lpa_engine *e, uint64_t n, const uint64_t *row, const uint32_t *col,
                          uint32_t small_thresh);
+/* BFS relabelling of a CSR on the device: host arrays row[n+1], col[row[n]]
+ * (ids < n) in; new_row[n+1], new_col[row[n]] and perm[n] (perm[v] = the new
+ * id of v) out.  New ids follow a breadth-first search from `root` taking
+ * neighbours in ascending order, then the unreached vertices by id; every
+ * relabelled row is sorted.  Replaces the host's lpah_csr_bfs_relabel
+ * (include/lpa_host.h), with the same result bit for bit; the reference
+ * takes its vertex order from the input file (src/file.cpp:115-195) and has
+ * no relabelling of its own. */
+int lpa_csr_bfs_relabel(lpa_engine *e, uint64_t n, const uint64_t *row, const uint32_t *col, uint32_t root,
+                        uint64_t *new_row, uint32_t *new_col, uint32_t *perm);
 /* Allocation-free variant of lpa_graph_encode_csr.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
@@ -249,8 +263,9 @@ int lpa_time_iterations(lpa_engine *e, int method, int iters, int use_graph,
  * (on != 0), or launches them eagerly (0, the default). */
 int lpa_set_graph_mode(lpa_engine *e, int on);
 /* Times `iters` fused steps for each candidate schedule (lane, wave and
- * quarter-wave thresholds), keeps the fastest (median), and restores the
- * labels after every candidate, so tuning never changes results. */
+ * quarter-wave thresholds, then the bin order and XCD tile mapping for the
+ * best thresholds), keeps the fastest (median), and restores the labels
+ * after every candidate, so tuning never changes results. */
 int lpa_autotune(lpa_engine *e, int iters, lpa_sched_params *best, float *best_ms);
 /* Average time (ms) per step of each kernel family in the last eager timed
  * run (HIP events on the engine stream). */

@@ -6,5 +6,5 @@ Python binding of the two in-tree native libraries:
 """
 from . import host  # noqa: F401
 from .host import Cgr, Csr, LpaError, gen_copying, gen_lattice, gen_rmat, threads  # noqa: F401
-from .engine import Engine, Mlp, device_count  # noqa: F401,E402
+from .engine import Engine, Mlp, device_count, gen_rmat_device_bfs  # noqa: F401,E402
 from . import schedule  # noqa: F401,E402

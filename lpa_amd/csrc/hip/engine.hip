@@ -3,9 +3,9 @@
 //
 // Device layout after scheduling (all HBM-resident, 64-bit offsets):
 //   words   CGR stream as 64-bit words, records reordered so every bin is a
-//           contiguous run of positions, sorted by degree (descending)
-//           inside the bin; 8 zero words of padding for the readers'
-//           look-ahead
+//           contiguous run of positions, by degree (descending; or in
+//           vertex-id order) inside the bin; 8 zero words of padding for
+//           the readers' look-ahead
 //   offs    n_c+1 start bit offsets of the reordered records
 //   vid     vertex id of each position
 //   srow/snbr/svid   small-vertex CSR (deg < small threshold), same order rule
@@ -24,6 +24,7 @@
 #include <string>
 #include <vector>
 
+#include "bfs_relabel.hpp"
 #include "encode_kernels.hpp"
 #include "heavy_kernels.hpp"
 #include "mlp_kernels.hpp"
@@ -113,6 +114,9 @@ struct HeavySet {
     bool tagged = false;                     // exact: generation-tagged tables (see TABLE_GENS)
     DevBuf<uint32_t> epoch;                  // exact: passes completed
 };
+
+// XCDs of an MI355X (gfx950): workgroups are dealt to them round robin
+constexpr uint32_t XCDS = 8;
 
 struct Bin {
     int kind;          // LPA_BIN_*
@@ -265,6 +269,7 @@ struct lpa_engine {
         std::memcpy(a.cms_ab, cms_ab, sizeof cms_ab);
         a.changed = counter.p;
         a.rec = rec;
+        a.xcds = sched_used.xcd_tiles == 1 ? XCDS : 1;
         return a;
     }
 
@@ -837,6 +842,27 @@ int lpa_encode_csr_device(lpa_engine *e, uint64_t n, uint64_t n_c, const uint64_
     });
 }
 
+int lpa_csr_bfs_relabel(lpa_engine *e, uint64_t n, const uint64_t *row, const uint32_t *col, uint32_t root,
+                        uint64_t *new_row, uint32_t *new_col, uint32_t *perm) {
+    return guard([&] {
+        if (!e || !row || !new_row || (n && (!perm || (row[n] && (!col || !new_col)))))
+            throw Fail(LPA_ERR_ARG, "null argument");
+        if (n >= (1ull << 30)) throw Fail(LPA_ERR_RANGE, "n must be < 2^30");
+        if (row[0] != 0) throw Fail(LPA_ERR_FORMAT, "row[0] != 0");
+        for (uint64_t v = 0; v < n; v++)
+            if (row[v + 1] < row[v]) throw Fail(LPA_ERR_FORMAT, "row offsets decrease");
+        if (n && root >= n) throw Fail(LPA_ERR_RANGE, "root >= n");
+        HIPCHK(hipSetDevice(e->device));
+        try {
+            bfs_relabel_device(e->stream, n, row, col, root, new_row, new_col, perm);
+        } catch (const std::invalid_argument &x) {
+            throw Fail(LPA_ERR_FORMAT, x.what());
+        } catch (const std::runtime_error &x) {
+            throw Fail(LPA_ERR_HIP, x.what());
+        }
+    });
+}
+
 int lpa_graph_stream(lpa_engine *e, uint64_t *n_c, uint64_t *nbits, uint64_t *words, uint64_t *offsets,
                      uint32_t *vertices) {
     return guard([&] {
@@ -1089,10 +1115,13 @@ static void build_layout(lpa_engine *e, const std::vector<uint8_t> &kind_c, cons
         std::iota(order.begin(), order.end(), 0u);
         std::vector<int> sub(cnt);
         for (uint64_t i = 0; i < cnt; i++) sub[i] = sub_of(kinds[i], deg(i));
+        // inside a bin: degree descending (the default), or file order
+        // (vertex ids); measured alike on R-MAT 27 (within 1.5 %)
+        const bool by_degree = p.bin_order != 1;
         std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
             if (kinds[x] != kinds[y]) return kinds[x] < kinds[y];
             if (sub[x] != sub[y]) return sub[x] < sub[y];
-            return deg(x) > deg(y);
+            return by_degree && deg(x) > deg(y);
         });
         for (uint64_t q = 0; q < cnt;) {
             uint64_t r = q;
@@ -1180,6 +1209,7 @@ int lpa_schedule_degree(lpa_engine *e, const lpa_sched_params *p) {
         if (!e || !p) throw Fail(LPA_ERR_ARG, "null argument");
         if (!e->have_graph) throw Fail(LPA_ERR_STATE, "no graph uploaded");
         if (p->lane_max > 64) throw Fail(LPA_ERR_RANGE, "lane_max must be <= 64");
+        if (p->bin_order > 2 || p->xcd_tiles > 2) throw Fail(LPA_ERR_RANGE, "bin_order and xcd_tiles must be 0, 1 or 2");
         HIPCHK(hipSetDevice(e->device));
         lpa_sched_params q = *p;
         if (q.wave_max == 0) q.wave_max = 512;
@@ -1957,6 +1987,29 @@ int lpa_autotune(lpa_engine *e, int iters, lpa_sched_params *best, float *best_m
                         best_t = med;
                         bp = q;
                     }
+                }
+            }
+        }
+        // then the layout options for those thresholds: vertex order inside
+        // the bins and the XCD tile mapping
+        const lpa_sched_params base = bp;
+        for (uint32_t order : {1u, 2u}) {
+            for (uint32_t xcd : {1u, 2u}) {
+                lpa_sched_params q = base;
+                q.bin_order = order;
+                q.xcd_tiles = xcd;
+                int rc = lpa_schedule_degree(e, &q);
+                if (rc) throw Fail(rc, g_err);
+                rc = lpa_time_iterations(e, 0, iters, 0, ms.data(), nullptr);
+                if (rc) throw Fail(rc, g_err);
+                e->cur = saved_cur;
+                HIPCHK(hipMemcpy(e->L[e->cur].p, saved.data(), tot * 4, hipMemcpyHostToDevice));
+                std::vector<float> sorted(ms);
+                std::sort(sorted.begin(), sorted.end());
+                const float med = sorted[sorted.size() / 2];
+                if (med < best_t) {
+                    best_t = med;
+                    bp = q;
                 }
             }
         }

@@ -63,7 +63,19 @@ struct StepArgs {
     uint32_t begin, end;                   // position range of this bin
     // work recorder (null: off): lane-cycles spent per vertex id
     unsigned long long *__restrict__ rec;
+    uint32_t xcds;                         // > 1: tiles dealt per XCD (xcd_tile), else in workgroup order
 };
+
+// Workgroup b of a launch runs on XCD b mod X (the dispatcher deals
+// workgroups to the XCDs round robin, and each XCD has its own L2).  The
+// tile a workgroup takes: XCD x's workgroups take the contiguous tiles
+// [x q + min(x, r), ...) of the n = X q + r tiles, so the workgroups sharing
+// an L2 work on neighbouring vertices (a bijection of [0, n)).
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n, uint32_t X) {
+    if (X <= 1) return b;
+    const uint32_t x = b % X, i = b / X, q = n / X, r = n % X;
+    return x * q + (x < r ? x : r) + i;
+}
 
 // shader clock (cycles)
 __device__ __forceinline__ uint64_t cycles() { return __builtin_readcyclecounter(); }
@@ -168,7 +180,7 @@ __global__ void __launch_bounds__(BLOCK) k_lane(StepArgs a) {
     __shared__ uint32_t rows[STAGE ? N * BLOCK : 1];
     __shared__ uint32_t sv[BLOCK], sd[BLOCK];
     __shared__ uint64_t sbase[CSR ? BLOCK : 1];
-    const uint32_t first = a.begin + blockIdx.x * BLOCK;
+    const uint32_t first = a.begin + xcd_tile(blockIdx.x, gridDim.x, a.xcds) * BLOCK;
     const uint32_t p = first + threadIdx.x;
     const uint32_t nact = a.end - first < (uint32_t)BLOCK ? a.end - first : (uint32_t)BLOCK;
     const uint64_t t0 = a.rec ? cycles() : 0;
@@ -565,7 +577,7 @@ __global__ void __launch_bounds__(BLOCK) k_group(StepArgs a, uint32_t logT, uint
     GroupShared<G> *sh = &gsh[gid];
     const uint32_t stride = gridDim.x * NG;
     uint32_t nch = 0;
-    for (uint32_t p0 = a.begin + blockIdx.x * NG; p0 < a.end; p0 += stride) {
+    for (uint32_t p0 = a.begin + xcd_tile(blockIdx.x, gridDim.x, a.xcds) * NG; p0 < a.end; p0 += stride) {
         // all groups of a wave (or workgroup) run the same trip count
         const uint32_t p = p0 + gid;
         const bool live = p < a.end;

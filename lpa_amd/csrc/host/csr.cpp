@@ -164,17 +164,24 @@ int lpah_csr_from_arrays(uint64_t n, const uint64_t *row, const uint32_t *col, l
     return guard([&] {
         if (!out || !row) throw Error(E_ARG, "null argument");
         if (row[0] != 0) throw Error(E_FORMAT, "row[0] != 0");
-        for (uint64_t v = 0; v < n; v++) {
+        for (uint64_t v = 0; v < n; v++)
             if (row[v + 1] < row[v]) throw Error(E_FORMAT, "row offsets decrease");
+        int bad = 0;   // 1: id >= n, 2: row not strictly ascending
+#pragma omp parallel for schedule(dynamic, 4096) reduction(max : bad)
+        for (uint64_t v = 0; v < n; v++)
             for (uint64_t e = row[v]; e < row[v + 1]; e++) {
-                if (col[e] >= n) throw Error(E_RANGE, "neighbour id >= n");
-                if (e > row[v] && col[e] <= col[e - 1]) throw Error(E_FORMAT, "row not strictly ascending");
+                if (col[e] >= n) bad = std::max(bad, 1);
+                else if (e > row[v] && col[e] <= col[e - 1]) bad = std::max(bad, 2);
             }
-        }
+        if (bad == 1) throw Error(E_RANGE, "neighbour id >= n");
+        if (bad == 2) throw Error(E_FORMAT, "row not strictly ascending");
         auto *g = new lpah_csr;
         g->n = n;
         g->row.assign(row, row + n + 1);
-        g->col.assign(col, col + row[n]);
+        g->col.resize(row[n]);
+#pragma omp parallel for schedule(static)
+        for (uint64_t b = 0; b < row[n]; b += 1 << 20)
+            std::copy(col + b, col + std::min<uint64_t>(row[n], b + (1 << 20)), g->col.begin() + b);
         *out = g;
     });
 }

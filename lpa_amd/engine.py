@@ -8,7 +8,7 @@ import os
 
 import numpy as np
 
-from .host import Cgr, LpaError
+from .host import Cgr, Csr, LpaError, gen_rmat
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
@@ -60,12 +60,14 @@ EXPORTS = {
     "lpa_graph_stream": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP]),
     "lpa_encode_csr_device": (C.c_int, [_VP, C.c_uint64, C.c_uint64, _VP, _VP, _VP, _VP, _VP, C.c_uint64,
                                         C.POINTER(C.c_uint64)]),
+    "lpa_csr_bfs_relabel": (C.c_int, [_VP, C.c_uint64, _VP, _VP, C.c_uint32, _VP, _VP, _VP]),
 }
 
 
 class SchedParams(C.Structure):
     _fields_ = [("lane_max", C.c_uint32), ("wave_max", C.c_uint32),
-                ("block_max", C.c_uint32), ("relayout", C.c_uint32), ("quarter_max", C.c_uint32)]
+                ("block_max", C.c_uint32), ("relayout", C.c_uint32), ("quarter_max", C.c_uint32),
+                ("bin_order", C.c_uint32), ("xcd_tiles", C.c_uint32)]
 
 
 def library_path():
@@ -146,6 +148,19 @@ class Engine:
         _check(self._L.lpa_graph_encode_csr(self._h, n, _p(row), _p(col), small_thresh))
         self.n = n
 
+    def bfs_relabel(self, row, col, root=0):
+        """BFS relabelling of a CSR on the device: (new_row, new_col, perm),
+        perm[v] = new id of v -- the host's Csr.bfs_relabel, bit for bit"""
+        row = np.ascontiguousarray(row, np.uint64)
+        col = np.ascontiguousarray(col, np.uint32)
+        n = len(row) - 1
+        new_row = np.zeros(n + 1, np.uint64)
+        new_col = np.zeros(max(len(col), 1), np.uint32)
+        perm = np.zeros(max(n, 1), np.uint32)
+        _check(self._L.lpa_csr_bfs_relabel(self._h, n, _p(row), _p(col if len(col) else new_col), root,
+                                           _p(new_row), _p(new_col), _p(perm)))
+        return new_row, new_col[:len(col)], perm[:n]
+
     def stream(self):
         """(words u64 [ceil(nbits/64)], offsets u64 [n_c+1], vertices u32 [n], nbits)"""
         n_c, nbits = C.c_uint64(0), C.c_uint64(0)
@@ -184,8 +199,8 @@ class Engine:
         _check(self._L.lpa_lp_config(self._h, K, int(semi), cms_thresh, cms_depth, cms_width, cms_seed))
         self.K = K
 
-    def schedule_degree(self, lane_max=32, wave_max=512, block_max=0, quarter_max=128):
-        p = SchedParams(lane_max, wave_max, block_max, 1, quarter_max)
+    def schedule_degree(self, lane_max=32, wave_max=512, block_max=0, quarter_max=128, bin_order=0, xcd_tiles=0):
+        p = SchedParams(lane_max, wave_max, block_max, 1, quarter_max, bin_order, xcd_tiles)
         _check(self._L.lpa_schedule_degree(self._h, C.byref(p)))
 
     def autotune(self, iters=3):
@@ -193,7 +208,8 @@ class Engine:
         p = SchedParams()
         ms = C.c_float(0)
         _check(self._L.lpa_autotune(self._h, iters, C.byref(p), C.byref(ms)))
-        return dict(lane_max=p.lane_max, wave_max=p.wave_max, quarter_max=p.quarter_max), ms.value
+        return dict(lane_max=p.lane_max, wave_max=p.wave_max, quarter_max=p.quarter_max, bin_order=p.bin_order,
+                    xcd_tiles=p.xcd_tiles), ms.value
 
     def set_graph_mode(self, on=True):
         _check(self._L.lpa_set_graph_mode(self._h, int(on)))
@@ -352,3 +368,17 @@ class Mlp:
         losses = np.zeros(epochs, np.float32)
         _check(self._L.lpa_mlp_train(self._h, _p(x), _p(t), _p(m), len(x), epochs, batch, lr, seed, _p(losses)))
         return losses
+
+
+def gen_rmat_device_bfs(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, device=0):
+    """R-MAT graph drawn on the host (gen_rmat with bfs=False) and
+    BFS-relabelled from vertex 0 on the device: the graph gen_rmat(...,
+    bfs=True) builds, bit for bit, without the host's BFS"""
+    raw = gen_rmat(scale, edge_factor, a, b, c, seed=seed, bfs=False)
+    e = Engine(device)
+    try:
+        row, col, _ = e.bfs_relabel(raw.row, raw.col, 0)
+    finally:
+        e.close()
+    del raw
+    return Csr.from_arrays(row, col)

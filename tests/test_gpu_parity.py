@@ -23,7 +23,8 @@ def test_reference_file_sets(case):
 
 @pytest.mark.parametrize("sched", [dict(lane_max=32, wave_max=256), dict(lane_max=0, wave_max=0, block_max=1),
                                    dict(lane_max=0, wave_max=512), dict(lane_max=8, wave_max=32, block_max=64),
-                                   dict(lane_max=64, wave_max=512)])
+                                   dict(lane_max=64, wave_max=512), dict(bin_order=1, xcd_tiles=1),
+                                   dict(bin_order=2, xcd_tiles=1)])
 def test_each_kernel_family(sched):
     g = L.gen_rmat(13, 16, seed=11)
     run_and_compare(g.encode(0, 128), 4, **sched)
