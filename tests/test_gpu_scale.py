@@ -18,6 +18,7 @@ import time
 
 import numpy as np
 import pytest
+import xxhash
 
 import lpa_amd as L
 import pyoracle as O
@@ -26,6 +27,15 @@ from gpu_common import make_engine, oracle_view, run_and_compare
 pytestmark = pytest.mark.gpu
 
 CMS22 = dict(cms_thresh=4096, cms_depth=4, cms_width=1024, cms_seed=0)
+# R-MAT scale 27, edge factor 32, seed 4, BFS order: xxh3_64 of row then col
+RMAT27_CHECKSUM = "99d290a7c788d9c0"
+
+
+def checksum(g):
+    h = xxhash.xxh3_64()
+    h.update(np.ascontiguousarray(g.row).tobytes())
+    h.update(np.ascontiguousarray(g.col).tobytes())
+    return h.hexdigest()
 
 
 def log(msg):
@@ -100,9 +110,12 @@ def test_rmat27_k8_one_step_checks():
     """R-MAT scale 27, edge factor 32, seed 4, BFS order; K = 8 from L0 = v;
     CMS 4 x 2048 above degree 8192; ten steps"""
     t0 = time.perf_counter()
-    g = L.gen_rmat(27, 32, 0.57, 0.19, 0.19, seed=4, bfs=True)
+    g = L.gen_rmat_device_bfs(27, 32, 0.57, 0.19, 0.19, seed=4)
+    log(f"R-MAT 27: n={g.n} m={g.m} (BFS relabelled on the device) in {time.perf_counter() - t0:.0f} s; "
+        f"checksum {checksum(g)}")
+    assert checksum(g) == RMAT27_CHECKSUM
     cgr = g.encode(0, 128)
-    log(f"R-MAT 27: n={g.n} m={g.m} bits={cgr.nbits} built in {time.perf_counter() - t0:.0f} s")
+    log(f"R-MAT 27: bits={cgr.nbits} built in {time.perf_counter() - t0:.0f} s")
     del g
     one_step_checks(cgr, 10, (0, 9), 8, cms_thresh=8192, cms_depth=4, cms_width=2048, cms_seed=0)
 
@@ -111,7 +124,8 @@ def test_rmat24_k64_semi_one_step_checks():
     """R-MAT scale 24, edge factor 16, seed 3, BFS order; K = 64 vectors, each
     with 1 % of the vertices seeded from 100 classes (clamped), the rest
     unlabelled; exact mode; fifteen steps"""
-    g = L.gen_rmat(24, 16, 0.57, 0.19, 0.19, seed=3, bfs=True)
+    g = L.gen_rmat_device_bfs(24, 16, 0.57, 0.19, 0.19, seed=3)
+    assert checksum(g) == "d50d1452d707001b"   # the host-relabelled graph's (test_generators.py)
     cgr = g.encode(0, 128)
     n, K = g.n, 64
     del g
