@@ -160,6 +160,9 @@ def main():
                     help="also time CGR-encoding the workload's CSR on the device")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the headline phase (for profiling): no K=1, two-kernel or degree-binned K runs")
+    ap.add_argument("--stream-host", type=int, default=0, metavar="BATCH_BYTES",
+                    help="out-of-core: keep the CGR stream in pinned host memory and copy it to the device in "
+                         "batches of at most BATCH_BYTES every step (host-link-inclusive rate)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="finish with three plain device copies of 256 MiB (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -181,6 +184,8 @@ def main():
         log(f"device CGR encode {dev_enc:.3f}s")
 
     e = L.Engine(0)
+    if args.stream_host:
+        e.stream_residence(True, args.stream_host)
     e.upload(cgr)
     res = {}
     semi = bool(w.get("semi"))
@@ -285,7 +290,11 @@ def main():
                    "note": "CGR encode of the workload's CSR; device time includes the CSR upload"},
         **res,
     }
-    tr = measured_traffic(args.workload, K, dom)
+    if args.stream_host:
+        nb, stage = e.stream_batches()
+        line["config"]["stream"] = (f"host-resident (out-of-core): {nb} batches of at most {args.stream_host} bytes, "
+                                    f"{stage} bytes of staging; the rate includes the host-to-device copies")
+    tr = None if args.stream_host else measured_traffic(args.workload, K, dom)
     if tr:
         line["roofline"]["traffic"] = tr["bytes"]
         line["roofline"]["traffic_source"] = tr["source"]

@@ -260,8 +260,33 @@ int lpa_schedule_learned_judged(lpa_engine *e, lpa_mlp *m, int iters, float *lea
 int lpa_time_iterations(lpa_engine *e, int method, int iters, int use_graph,
                         float *ms, uint64_t *changed);
 /* Replays steps from HIP graphs captured once per label-buffer parity
- * (on != 0), or launches them eagerly (0, the default). */
+ * (on != 0), or launches them eagerly (0, the default).  Refused
+ * (LPA_ERR_STATE) while the stream is host-resident. */
 int lpa_set_graph_mode(lpa_engine *e, int on);
+
+/* where the CGR stream lives (lpa_stream_residence) */
+enum {
+    LPA_STREAM_DEVICE = 0,  /* in HBM (the default) */
+    LPA_STREAM_HOST = 1     /* in pinned host memory, streamed to the device every step */
+};
+/* Out-of-core operation for graphs whose CGR stream does not fit in HBM next
+ * to the labels.  With LPA_STREAM_HOST the stream (as uploaded and as
+ * scheduled) is kept in pinned host memory; every fused or two-kernel step
+ * copies the scheduled stream to the device in batches of whole records of
+ * at most batch_bytes (0: 1 GiB; a longer record is a batch of its own) into
+ * two HBM staging buffers, copying batch i+1 on a second stream while batch
+ * i's kernels run.  Offsets, segment metadata, labels, the heavy vertices'
+ * decoded lists and the small-vertex CSR stay in HBM; results are those of
+ * the HBM-resident stream.  Steps are not captured in graphs meanwhile.  May
+ * be called before or after uploading and scheduling: it moves the stream
+ * and plans the batches of the current layout (every later layout is
+ * planned in turn).  Replaces OutOfCore (include/outofcore.cuh:9-96:
+ * compute_batch_boundaries, transfer_batch, transfer_next_batch), which
+ * batches the neighbour lists of an uncompressed CSR the same way. */
+int lpa_stream_residence(lpa_engine *e, int where, uint64_t batch_bytes);
+/* Batches of the current layout and the bytes of both staging buffers
+ * (0 and 0 while the stream is in HBM). */
+int lpa_stream_batches(lpa_engine *e, uint64_t *nbatches, uint64_t *stage_bytes);
 /* Times `iters` fused steps for each candidate schedule (lane, wave and
  * quarter-wave thresholds, then the bin order and XCD tile mapping for the
  * best thresholds), keeps the fastest (median), and restores the labels

@@ -69,20 +69,39 @@ template <class F> int guard(F &&f) {
 template <class T> struct DevBuf {
     T *p = nullptr;
     uint64_t n = 0;
+    bool host = false;   // pinned host memory, mapped into the device's address space
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) (void)(host ? hipHostFree(p) : hipFree(p));
         p = nullptr;
         n = 0;
+        host = false;
     }
     void alloc(uint64_t cnt) {
         release();
         if (cnt == 0) cnt = 1;
         HIPCHK(hipMalloc((void **)&p, cnt * sizeof(T)));
         n = cnt;
+    }
+    // pinned host memory that kernels can read in place (same address on
+    // the device) and copies can stream from at full link speed
+    void alloc_host(uint64_t cnt) {
+        release();
+        if (cnt == 0) cnt = 1;
+        HIPCHK(hipHostMalloc((void **)&p, cnt * sizeof(T), hipHostMallocDefault));
+        host = true;
+        n = cnt;
+        void *d = nullptr;
+        HIPCHK(hipHostGetDevicePointer(&d, p, 0));
+        if (d != (void *)p) throw Fail(LPA_ERR_HIP, "pinned host memory is mapped at another device address");
+    }
+    void swap(DevBuf &o) {
+        std::swap(p, o.p);
+        std::swap(n, o.n);
+        std::swap(host, o.host);
     }
     void upload(const T *h, uint64_t cnt, hipStream_t s) {
         alloc(cnt);
@@ -113,7 +132,21 @@ struct HeavySet {
     uint32_t stash_blocks = 0;
     bool tagged = false;                     // exact: generation-tagged tables (see TABLE_GENS)
     DevBuf<uint32_t> epoch;                  // exact: passes completed
+    std::vector<uint32_t> task_pos;          // stream position of every decode task (ascending)
 };
+
+// One batch of a host-resident stream: stream positions [p0, p1) of the
+// compressed part, whose records (and the readers' 8 look-ahead words) lie
+// in words [w0, w0 + nw), and per heavy set the decode tasks [first,
+// second) of those records.
+struct StreamBatch {
+    uint32_t p0 = 0, p1 = 0;
+    uint64_t w0 = 0, nw = 0;
+    std::vector<std::pair<uint32_t, uint32_t>> tasks;
+};
+
+// default bytes of one staging buffer of a host-resident stream
+constexpr uint64_t DEFAULT_BATCH_BYTES = 1ull << 30;
 
 // XCDs of an MI355X (gfx950): workgroups are dealt to them round robin
 constexpr uint32_t XCDS = 8;
@@ -218,6 +251,16 @@ struct lpa_engine {
     DevBuf<uint32_t> ccol;
     bool have_csr = false;
 
+    // host-resident stream (lpa_stream_residence): words0 and words in pinned
+    // host memory; the fused and two-kernel steps copy the scheduled stream
+    // batch by batch into stage[0] / stage[1] on copy_stream
+    bool host_stream = false;
+    uint64_t batch_bytes = DEFAULT_BATCH_BYTES;
+    std::vector<StreamBatch> batches;
+    DevBuf<uint64_t> stage[2];
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t ev_copied[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
+
     // timing of the last timed run, per bin kind
     float kind_ms[LPA_NBIN_KINDS] = {0};
 
@@ -242,6 +285,9 @@ struct lpa_engine {
     ~lpa_engine() {
         drop_graphs();
         if (h_counter) (void)hipHostFree(h_counter);
+        for (hipEvent_t x : {ev_copied[0], ev_copied[1], ev_free[0], ev_free[1]})
+            if (x) (void)hipEventDestroy(x);
+        if (copy_stream) (void)hipStreamDestroy(copy_stream);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -274,9 +320,13 @@ struct lpa_engine {
     }
 
     void launch_bin(const Bin &b, StepArgs a, bool as_csr);
-    void launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr);
-    void step(bool cglp, hipEvent_t *kind_events);
-    void run_heavy(bool cglp, hipEvent_t *kind_events, bool *started);
+    HeavyArgs heavy_args(const HeavySet &hs, const StepArgs &a, bool as_csr) const;
+    void launch_heavy_decode(const HeavySet &hs, const StepArgs &a, uint32_t t0, uint32_t t1);
+    void launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr, bool decode = true);
+    template <class F> void for_each_batch(F &&fn);
+    void step_host_stream();
+    void step(bool cglp, hipEvent_t *launch_events);
+    void run_heavy(bool cglp, hipEvent_t *launch_events);
     void step_bin_times(std::vector<float> &bin_ms, std::vector<float> &heavy_ms);
 };
 
@@ -391,8 +441,7 @@ static size_t cms_local_lds(uint32_t kbl, size_t cells) {
     return (kbl == 1 ? 1 : (kbl + 1) / 2) * cells * sizeof(uint32_t);
 }
 
-void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
-    if (hs.nh == 0) return;
+HeavyArgs lpa_engine::heavy_args(const HeavySet &hs, const StepArgs &a, bool as_csr) const {
     HeavyArgs h{};
     h.s = a;
     h.nh = hs.nh;
@@ -413,17 +462,30 @@ void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
     h.epoch = hs.epoch.p;
     h.tagged = hs.tagged ? 1 : 0;
     h.kb_alloc = hs.kb;
-    // CMS: sketches of up to KB label vectors side by side in LDS
+    // CMS: sketches of up to KB label vectors side by side in LDS; both CMS
+    // kernels pack two vectors' 16-bit LDS cells in a word when K > 1 (a
+    // list of <= CMS_LOCAL ids, or a chunk, cannot overflow one)
+    h.kbs = cms_local_kb(K, (size_t)cms_depth * cms_width, lds_max - 20480);
+    h.kbl = h.kbs;
+    return h;
+}
+
+// Decode tasks [t0, t1) of a heavy set (compressed part) into its lists.
+void lpa_engine::launch_heavy_decode(const HeavySet &hs, const StepArgs &a, uint32_t t0, uint32_t t1) {
+    if (t1 <= t0) return;
+    const HeavyArgs h = heavy_args(hs, a, false);
+    if (hs.cms) hipLaunchKernelGGL(k_heavy_decode<1>, dim3(t1 - t0), dim3(BLOCK), 0, stream, h, hs.tasks.p + t0, t1 - t0);
+    else hipLaunchKernelGGL(k_heavy_decode<0>, dim3(t1 - t0), dim3(BLOCK), 0, stream, h, hs.tasks.p + t0, t1 - t0);
+}
+
+// A heavy set's step: decode its records into the lists (unless `decode` is
+// off: a host-resident stream decodes them batch by batch), then count.
+void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr, bool decode) {
+    if (hs.nh == 0) return;
+    HeavyArgs h = heavy_args(hs, a, as_csr);
     const bool one = K == 1;
     const size_t cells = (size_t)cms_depth * cms_width;
-    // both CMS kernels pack two vectors' 16-bit LDS cells in a word when
-    // K > 1 (a list of <= CMS_LOCAL ids, or a chunk, cannot overflow one)
-    h.kbs = cms_local_kb(K, cells, lds_max - 20480);
-    h.kbl = h.kbs;
-    if (!as_csr) {
-        if (hs.cms) hipLaunchKernelGGL(k_heavy_decode<1>, dim3(hs.ntasks), dim3(BLOCK), 0, stream, h, hs.tasks.p, hs.ntasks);
-        else hipLaunchKernelGGL(k_heavy_decode<0>, dim3(hs.ntasks), dim3(BLOCK), 0, stream, h, hs.tasks.p, hs.ntasks);
-    }
+    if (!as_csr && decode) launch_heavy_decode(hs, a, 0, hs.ntasks);
     if (hs.nlocal) {
         const dim3 b(HBLOCK);
         const size_t sk = cms_local_lds(h.kbl, cells);
@@ -480,37 +542,86 @@ void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr) {
 
 // One synchronous step: clear the counter, every bin reads L[cur] and writes
 // L[cur^1].  Vertices without neighbours are never written (both buffers
-// hold their initial label).
-void lpa_engine::step(bool cglp, hipEvent_t *kind_events) {
+// hold their initial label).  launch_events (or null): 2 * (bins + heavy
+// sets) events, recorded around bin i at [2i, 2i+1] and around heavy set j
+// at [2(nbins + j), ...], so a family's time is the sum over its launches.
+void lpa_engine::step(bool cglp, hipEvent_t *launch_events) {
+    if (host_stream && !cglp) {
+        step_host_stream();
+        return;
+    }
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, stream, counter.p, (uint64_t)1, 0ull);
     StepArgs a = base_args(cglp);
-    bool started[LPA_NBIN_KINDS] = {false};
-    for (const Bin &b : bins) {
+    for (size_t i = 0; i < bins.size(); i++) {
+        const Bin &b = bins[i];
         if (b.kind == LPA_BIN_SKIP || b.kind == LPA_BIN_CMS || b.kind == LPA_BIN_GLOBAL || b.end == b.begin) continue;
-        if (kind_events && !started[b.kind]) {
-            HIPCHK(hipEventRecord(kind_events[2 * b.kind], stream));
-            started[b.kind] = true;
-        }
+        if (launch_events) HIPCHK(hipEventRecord(launch_events[2 * i], stream));
         launch_bin(b, a, b.csr || cglp);
-        if (kind_events) HIPCHK(hipEventRecord(kind_events[2 * b.kind + 1], stream));
+        if (launch_events) HIPCHK(hipEventRecord(launch_events[2 * i + 1], stream));
     }
-    run_heavy(cglp, kind_events, started);
+    run_heavy(cglp, launch_events);
     HIPCHK(hipGetLastError());
 }
 
-void lpa_engine::run_heavy(bool cglp, hipEvent_t *kind_events, bool *started) {
-    for (auto &hp : heavy) {
-        HeavySet &hs = *hp;
+void lpa_engine::run_heavy(bool cglp, hipEvent_t *launch_events) {
+    for (size_t j = 0; j < heavy.size(); j++) {
+        HeavySet &hs = *heavy[j];
         if (hs.nh == 0) continue;
-        const int kind = hs.cms ? LPA_BIN_CMS : LPA_BIN_GLOBAL;
-        if (kind_events && !started[kind]) {
-            HIPCHK(hipEventRecord(kind_events[2 * kind], stream));
-            started[kind] = true;
-        }
+        hipEvent_t *ev = launch_events ? launch_events + 2 * (bins.size() + j) : nullptr;
+        if (ev) HIPCHK(hipEventRecord(ev[0], stream));
         const StepArgs a = base_args(cglp && !hs.csr_part);
         launch_heavy(hs, a, hs.csr_part || cglp);
-        if (kind_events) HIPCHK(hipEventRecord(kind_events[2 * kind + 1], stream));
+        if (ev) HIPCHK(hipEventRecord(ev[1], stream));
     }
+}
+
+// Runs fn(batch, words) for every batch of a host-resident stream, in
+// stream order on the engine stream.  Batch i is copied into stage[i % 2]
+// on the copy stream once the kernels of batch i - 2 (the buffer's last
+// user) are done, so the copy of batch i + 1 overlaps batch i's kernels;
+// `words` is the staging buffer rebased by the batch's first word, so the
+// kernels address the batch's records by their absolute stream positions.
+template <class F> void lpa_engine::for_each_batch(F &&fn) {
+    for (size_t i = 0; i < batches.size(); i++) {
+        const StreamBatch &b = batches[i];
+        const int s = (int)(i & 1);
+        HIPCHK(hipStreamWaitEvent(copy_stream, ev_free[s], 0));
+        HIPCHK(hipMemcpyAsync(stage[s].p, words.p + b.w0, b.nw * sizeof(uint64_t), hipMemcpyHostToDevice,
+                              copy_stream));
+        HIPCHK(hipEventRecord(ev_copied[s], copy_stream));
+        HIPCHK(hipStreamWaitEvent(stream, ev_copied[s], 0));
+        const auto *w = reinterpret_cast<const uint64_t *>(reinterpret_cast<uintptr_t>(stage[s].p) -
+                                                           b.w0 * sizeof(uint64_t));
+        fn(b, w);
+        HIPCHK(hipEventRecord(ev_free[s], stream));
+    }
+}
+
+// The fused step over a host-resident stream: per batch, every bin's share
+// of the batch's positions and the decode tasks of its heavy records; then
+// the small CSR's bins and the heavy counting, which read only HBM.  Every
+// launch reads L[cur] and writes L[cur ^ 1], so the order of the launches
+// changes nothing.
+void lpa_engine::step_host_stream() {
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, stream, counter.p, (uint64_t)1, 0ull);
+    const StepArgs a = base_args(false);
+    for_each_batch([&](const StreamBatch &sb, const uint64_t *w) {
+        StepArgs ab = a;
+        ab.words = w;
+        for (const Bin &b : bins) {
+            if (b.csr || b.kind == LPA_BIN_SKIP || b.kind == LPA_BIN_CMS || b.kind == LPA_BIN_GLOBAL) continue;
+            Bin c = b;
+            c.begin = std::max(b.begin, sb.p0);
+            c.end = std::min(b.end, sb.p1);
+            if (c.begin < c.end) launch_bin(c, ab, false);
+        }
+        for (size_t j = 0; j < heavy.size(); j++)
+            if (!heavy[j]->csr_part) launch_heavy_decode(*heavy[j], ab, sb.tasks[j].first, sb.tasks[j].second);
+    });
+    for (const Bin &b : bins)
+        if (b.csr && b.kind != LPA_BIN_SKIP && b.kind != LPA_BIN_CMS && b.kind != LPA_BIN_GLOBAL) launch_bin(b, a, true);
+    for (auto &hp : heavy) launch_heavy(*hp, a, hp->csr_part, false);
+    HIPCHK(hipGetLastError());
 }
 
 // One fused step like step(false, ...), timing every bin launch and every
@@ -588,6 +699,8 @@ static void finish_graph(lpa_engine *e) {
 static uint64_t lds_budget(const lpa_engine *e);
 static uint32_t wave_cap(const lpa_engine *e);
 static uint32_t block_cap(const lpa_engine *e);
+static void move_stream(lpa_engine *e, DevBuf<uint64_t> &b, bool to_host);
+static void plan_batches(lpa_engine *e);
 
 extern "C" {
 
@@ -699,11 +812,23 @@ int lpa_graph_upload(lpa_engine *e, uint64_t n, uint64_t n_c, const uint32_t *wo
         e->h_vertices.assign(vertices, vertices + n);
 
         const uint64_t n64 = (nbits + 63) / 64 + 8;
-        DevBuf<uint32_t> w32;
-        w32.upload(words, nwords, e->stream);
-        e->words0.alloc(n64);
-        hipLaunchKernelGGL(k_pack64, dim3((unsigned)((n64 + 255) / 256)), dim3(256), 0, e->stream, w32.p,
-                           std::min<uint64_t>(nwords, (nbits + 31) / 32), e->words0.p, n64);
+        const uint64_t n32 = std::min<uint64_t>(nwords, (nbits + 31) / 32);
+        if (e->host_stream) {
+            // packed on the host straight into pinned memory: the stream
+            // never occupies HBM
+            e->words0.alloc_host(n64);
+            for (uint64_t i = 0; i < n64; i++) {
+                const uint64_t hi = 2 * i < n32 ? words[2 * i] : 0, lo = 2 * i + 1 < n32 ? words[2 * i + 1] : 0;
+                e->words0.p[i] = (hi << 32) | lo;
+            }
+        } else {
+            DevBuf<uint32_t> w32;
+            w32.upload(words, nwords, e->stream);
+            e->words0.alloc(n64);
+            hipLaunchKernelGGL(k_pack64, dim3((unsigned)((n64 + 255) / 256)), dim3(256), 0, e->stream, w32.p, n32,
+                               e->words0.p, n64);
+            e->sync();
+        }
         e->offs0.upload(offsets, n_c + 1, e->stream);
         e->vid0.upload(vertices, n_c, e->stream);
         finish_graph(e);
@@ -785,6 +910,7 @@ int lpa_graph_encode_csr(lpa_engine *e, uint64_t n, const uint64_t *row, const u
             hipLaunchKernelGGL(k_encode_write, dim3((unsigned)((n_c + 255) / 256)), dim3(256), 0, e->stream, drow.p,
                                dcol.p, e->vid0.p, n_c, e->offs0.p, e->words0.p);
         HIPCHK(hipGetLastError());
+        if (e->host_stream) move_stream(e, e->words0, true);
         finish_graph(e);
     });
 }
@@ -872,7 +998,7 @@ int lpa_graph_stream(lpa_engine *e, uint64_t *n_c, uint64_t *nbits, uint64_t *wo
         *n_c = e->n_c;
         *nbits = e->nbits;
         if (words && e->nbits)
-            HIPCHK(hipMemcpyAsync(words, e->words0.p, (e->nbits + 63) / 64 * 8, hipMemcpyDeviceToHost, e->stream));
+            HIPCHK(hipMemcpyAsync(words, e->words0.p, (e->nbits + 63) / 64 * 8, hipMemcpyDefault, e->stream));
         if (offsets) std::memcpy(offsets, e->h_offs.data(), (e->n_c + 1) * 8);
         if (vertices) std::memcpy(vertices, e->h_vertices.data(), e->n * 4);
         e->sync();
@@ -1024,6 +1150,7 @@ static void build_heavy(lpa_engine *e) {
                         t.ns = segs;
                         tk.push_back(t);
                         owner.push_back(i);
+                        hs->task_pos.push_back(q);
                     }
                 }
                 hs->ntasks = (uint32_t)tk.size();
@@ -1084,6 +1211,110 @@ static void build_heavy(lpa_engine *e) {
     e->sync();
 }
 
+// ------------------------------------------------- host-resident stream
+
+static uint64_t *rebased(uint64_t *stage, uint64_t w0) {
+    return reinterpret_cast<uint64_t *>(reinterpret_cast<uintptr_t>(stage) - w0 * sizeof(uint64_t));
+}
+
+// Position batches of the scheduled stream (h_loffs): each takes whole
+// records while its words, the readers' 8 look-ahead words included, fit in
+// `cap` words; a longer record is a batch of its own.
+static std::vector<StreamBatch> stream_batches(const lpa_engine *e, uint64_t cap) {
+    std::vector<StreamBatch> out;
+    auto span_end = [&](uint64_t r) { return ((e->h_loffs[r] + 63) >> 6) + 8; };
+    for (uint64_t q = 0; q < e->n_c;) {
+        StreamBatch b;
+        b.p0 = (uint32_t)q;
+        b.w0 = e->h_loffs[q] >> 6;
+        uint64_t r = q + 1;
+        while (r < e->n_c && span_end(r + 1) - b.w0 <= cap) r++;
+        b.p1 = (uint32_t)r;
+        b.nw = span_end(r) - b.w0;
+        out.push_back(std::move(b));
+        q = r;
+    }
+    return out;
+}
+
+// Staging buffers (HBM) for the largest batch, the copy stream and its
+// events; both buffers start free.
+static void ensure_staging(lpa_engine *e, const std::vector<StreamBatch> &bs) {
+    uint64_t maxw = 1;
+    for (const auto &b : bs) maxw = std::max(maxw, b.nw);
+    e->sync();
+    if (e->copy_stream) HIPCHK(hipStreamSynchronize(e->copy_stream));
+    for (auto &s : e->stage)
+        if (s.n < maxw) s.alloc(maxw);
+    if (!e->copy_stream) {
+        HIPCHK(hipStreamCreateWithFlags(&e->copy_stream, hipStreamNonBlocking));
+        for (int s = 0; s < 2; s++) {
+            HIPCHK(hipEventCreateWithFlags(&e->ev_copied[s], hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&e->ev_free[s], hipEventDisableTiming));
+            HIPCHK(hipEventRecord(e->ev_free[s], e->stream));
+        }
+    }
+}
+
+// The batches of the current layout, with each heavy set's decode tasks.
+static void plan_batches(lpa_engine *e) {
+    auto bs = stream_batches(e, std::max<uint64_t>(e->batch_bytes / sizeof(uint64_t), 16));
+    for (auto &b : bs) {
+        b.tasks.resize(e->heavy.size());
+        for (size_t j = 0; j < e->heavy.size(); j++) {
+            const auto &tp = e->heavy[j]->task_pos;
+            b.tasks[j] = {(uint32_t)(std::lower_bound(tp.begin(), tp.end(), b.p0) - tp.begin()),
+                          (uint32_t)(std::lower_bound(tp.begin(), tp.end(), b.p1) - tp.begin())};
+        }
+    }
+    ensure_staging(e, bs);
+    e->batches = std::move(bs);
+}
+
+// Relayout into a host-resident stream, batch by batch: k_relayout ORs the
+// records into a zeroed staging buffer (atomically, for words shared by two
+// records; atomics are kept off the host link), then the batch's words are
+// copied back.  The first word of a batch may hold the previous batch's
+// last bits: it is ORed in on the host.
+static void relayout_to_host(lpa_engine *e, const uint32_t *perm) {
+    const uint64_t n64 = (e->nbits + 63) / 64 + 8;
+    e->words.alloc_host(n64);
+    const auto bs = stream_batches(e, std::max<uint64_t>(e->batch_bytes / sizeof(uint64_t), 16));
+    ensure_staging(e, bs);
+    uint64_t *st = e->stage[0].p;
+    uint64_t written = 0;   // host words [0, written) are filled; the last may be shared
+    for (const auto &b : bs) {
+        const uint64_t nw = ((e->h_loffs[b.p1] + 63) >> 6) - b.w0;   // words holding the batch's bits
+        const uint32_t cnt = b.p1 - b.p0;
+        if (nw == 0) continue;
+        HIPCHK(hipMemsetAsync(st, 0, nw * sizeof(uint64_t), e->stream));
+        hipLaunchKernelGGL(k_relayout, dim3((cnt + 255) / 256), dim3(256), 0, e->stream, e->words0.p, e->offs0.p,
+                           e->vid0.p, e->meta0.p, perm + b.p0, e->offs.p + b.p0, rebased(st, b.w0), e->vid.p + b.p0,
+                           e->meta.p + b.p0, (uint64_t)cnt);
+        HIPCHK(hipGetLastError());
+        uint64_t first = 0;
+        HIPCHK(hipMemcpyAsync(&first, st, sizeof first, hipMemcpyDeviceToHost, e->stream));
+        if (nw > 1)
+            HIPCHK(hipMemcpyAsync(e->words.p + b.w0 + 1, st + 1, (nw - 1) * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                  e->stream));
+        e->sync();
+        e->words.p[b.w0] = (b.w0 < written ? e->words.p[b.w0] : 0) | first;
+        written = b.w0 + nw;
+    }
+    std::memset(e->words.p + written, 0, (n64 - written) * sizeof(uint64_t));
+}
+
+// Moves a stream buffer between HBM and pinned host memory.
+static void move_stream(lpa_engine *e, DevBuf<uint64_t> &b, bool to_host) {
+    if (!b.p || b.host == to_host) return;
+    e->sync();
+    DevBuf<uint64_t> t;
+    if (to_host) t.alloc_host(b.n);
+    else t.alloc(b.n);
+    HIPCHK(hipMemcpy(t.p, b.p, b.n * sizeof(uint64_t), hipMemcpyDefault));
+    b.swap(t);
+}
+
 // Builds the device layout from a kind per file-order position (compressed
 // positions first, then small positions).
 static void build_layout(lpa_engine *e, const std::vector<uint8_t> &kind_c, const std::vector<uint8_t> &kind_s,
@@ -1139,18 +1370,22 @@ static void build_layout(lpa_engine *e, const std::vector<uint8_t> &kind_c, cons
             for (uint64_t q = 0; q < cnt; q++)
                 noffs[q + 1] = noffs[q] + (e->h_offs[order[q] + 1] - e->h_offs[order[q]]);
             const uint64_t n64 = (e->nbits + 63) / 64 + 8;
-            e->words.alloc(n64);
-            HIPCHK(hipMemsetAsync(e->words.p, 0, n64 * 8, e->stream));
             e->offs.upload(noffs.data(), cnt + 1, e->stream);
             e->h_loffs = noffs;
             e->vid.alloc(std::max<uint64_t>(cnt, 1));
             e->meta.alloc(std::max<uint64_t>(cnt, 1));
             DevBuf<uint32_t> perm;
             perm.upload(order.data(), cnt, e->stream);
-            if (cnt)
-                hipLaunchKernelGGL(k_relayout, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, e->stream,
-                                   e->words0.p, e->offs0.p, e->vid0.p, e->meta0.p, perm.p, e->offs.p, e->words.p,
-                                   e->vid.p, e->meta.p, cnt);
+            if (e->host_stream) {
+                relayout_to_host(e, perm.p);
+            } else {
+                e->words.alloc(n64);
+                HIPCHK(hipMemsetAsync(e->words.p, 0, n64 * 8, e->stream));
+                if (cnt)
+                    hipLaunchKernelGGL(k_relayout, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, e->stream,
+                                       e->words0.p, e->offs0.p, e->vid0.p, e->meta0.p, perm.p, e->offs.p,
+                                       e->words.p, e->vid.p, e->meta.p, cnt);
+            }
             HIPCHK(hipGetLastError());
             e->sync();
         } else {
@@ -1172,6 +1407,7 @@ static void build_layout(lpa_engine *e, const std::vector<uint8_t> &kind_c, cons
         }
     }
     build_heavy(e);
+    if (e->host_stream) plan_batches(e);
     e->have_csr = false;
     e->scheduled = true;
     (void)p;
@@ -1395,8 +1631,65 @@ int lpa_nova_iterate(lpa_engine *e, uint64_t *changed) {
 int lpa_set_graph_mode(lpa_engine *e, int on) {
     return guard([&] {
         if (!e) throw Fail(LPA_ERR_ARG, "null argument");
+        if (on && e->host_stream) throw Fail(LPA_ERR_STATE, "the stream is host-resident: steps are not captured");
         e->graph_mode = on != 0;
         if (!e->graph_mode) e->drop_graphs();
+    });
+}
+
+// CGLP's first kernel over stream positions [p0, p1): decode every
+// compressed record there into its CSR row.
+static void decode_range(lpa_engine *e, const uint64_t *words, uint32_t p0, uint32_t p1) {
+    for (const Bin &b : e->bins) {
+        if (b.csr || b.kind == LPA_BIN_SKIP) continue;
+        const uint32_t lo = std::max(b.begin, p0), hi = std::min(b.end, p1);
+        if (lo >= hi) continue;
+        const uint32_t count = hi - lo;
+        if (b.maxdeg <= 256)
+            hipLaunchKernelGGL(k_decode_csr, dim3((count + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, e->stream, words,
+                               e->offs.p, e->vid.p, e->crow.p, e->ccol.p, lo, hi);
+        else
+            hipLaunchKernelGGL(k_decode_csr_group, dim3(std::min<uint32_t>(count, 4096)), dim3(BLOCK), 0, e->stream,
+                               words, e->offs.p, e->vid.p, e->crow.p, e->ccol.p, lo, hi);
+    }
+}
+
+// Decodes the whole compressed part to the CSR (batch by batch when the
+// stream is host-resident).
+static void decode_all(lpa_engine *e) {
+    if (e->host_stream)
+        e->for_each_batch([&](const StreamBatch &sb, const uint64_t *w) { decode_range(e, w, sb.p0, sb.p1); });
+    else decode_range(e, e->words.p, 0, (uint32_t)e->n_c);
+    HIPCHK(hipGetLastError());
+}
+
+int lpa_stream_residence(lpa_engine *e, int where, uint64_t batch_bytes) {
+    return guard([&] {
+        if (!e) throw Fail(LPA_ERR_ARG, "null argument");
+        if (where != LPA_STREAM_DEVICE && where != LPA_STREAM_HOST)
+            throw Fail(LPA_ERR_ARG, "where: LPA_STREAM_DEVICE or LPA_STREAM_HOST");
+        const bool host = where == LPA_STREAM_HOST;
+        if (host && e->graph_mode) throw Fail(LPA_ERR_STATE, "graph mode is on: a host-resident stream is not captured");
+        HIPCHK(hipSetDevice(e->device));
+        e->sync();
+        if (e->copy_stream) HIPCHK(hipStreamSynchronize(e->copy_stream));
+        e->host_stream = host;
+        e->batch_bytes = batch_bytes ? batch_bytes : DEFAULT_BATCH_BYTES;
+        move_stream(e, e->words0, host);
+        move_stream(e, e->words, host);
+        if (host && e->scheduled) plan_batches(e);
+        if (!host) {
+            e->batches.clear();
+            for (auto &st : e->stage) st.release();
+        }
+    });
+}
+
+int lpa_stream_batches(lpa_engine *e, uint64_t *nbatches, uint64_t *stage_bytes) {
+    return guard([&] {
+        if (!e || !nbatches || !stage_bytes) throw Fail(LPA_ERR_ARG, "null argument");
+        *nbatches = e->host_stream ? e->batches.size() : 0;
+        *stage_bytes = e->host_stream ? 2 * e->stage[0].n * sizeof(uint64_t) : 0;
     });
 }
 
@@ -1406,17 +1699,7 @@ static void cglp_decode(lpa_engine *e) {
     for (uint64_t q = 0; q < e->n_c; q++) row[q + 1] = row[q] + e->deg_c[e->order_c[q]];
     e->crow.upload(row.data(), e->n_c + 1, e->stream);
     e->ccol.alloc(std::max<uint64_t>(row[e->n_c], 1));
-    for (const Bin &b : e->bins) {
-        if (b.csr || b.kind == LPA_BIN_SKIP || b.end == b.begin) continue;
-        const uint32_t count = b.end - b.begin;
-        if (b.maxdeg <= 256)
-            hipLaunchKernelGGL(k_decode_csr, dim3((count + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, e->stream,
-                               e->words.p, e->offs.p, e->vid.p, e->crow.p, e->ccol.p, b.begin, b.end);
-        else
-            hipLaunchKernelGGL(k_decode_csr_group, dim3(std::min<uint32_t>(count, 4096)), dim3(BLOCK), 0,
-                               e->stream, e->words.p, e->offs.p, e->vid.p, e->crow.p, e->ccol.p, b.begin, b.end);
-    }
-    HIPCHK(hipGetLastError());
+    decode_all(e);
     e->have_csr = true;
 }
 
@@ -1432,19 +1715,7 @@ int lpa_cglp_decode(lpa_engine *e) {
 // kernels over the CSR (small vertices read their own CSR as in NOVA).
 static void cglp_step(lpa_engine *e) {
     if (!e->have_csr) cglp_decode(e);
-    else {
-        for (const Bin &b : e->bins) {
-            if (b.csr || b.kind == LPA_BIN_SKIP || b.end == b.begin) continue;
-            const uint32_t count = b.end - b.begin;
-            if (b.maxdeg <= 256)
-                hipLaunchKernelGGL(k_decode_csr, dim3((count + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, e->stream,
-                                   e->words.p, e->offs.p, e->vid.p, e->crow.p, e->ccol.p, b.begin, b.end);
-            else
-                hipLaunchKernelGGL(k_decode_csr_group, dim3(std::min<uint32_t>(count, 4096)), dim3(BLOCK), 0,
-                                   e->stream, e->words.p, e->offs.p, e->vid.p, e->crow.p, e->ccol.p, b.begin,
-                                   b.end);
-        }
-    }
+    else decode_all(e);
     csr_lp(e);
 }
 
@@ -1460,8 +1731,7 @@ static void csr_lp(lpa_engine *e) {
         if (b.csr) e->launch_bin(b, a, true);
         else e->launch_bin(b, ac, true);
     }
-    bool started[LPA_NBIN_KINDS] = {false};
-    e->run_heavy(true, nullptr, started);
+    e->run_heavy(true, nullptr);
     HIPCHK(hipGetLastError());
 }
 
@@ -1906,14 +2176,33 @@ int lpa_time_iterations(lpa_engine *e, int method, int iters, int use_graph, flo
     return guard([&] {
         require_ready(e);
         if (iters <= 0 || !ms) throw Fail(LPA_ERR_ARG, "bad arguments");
-        hipEvent_t ev[2];
-        hipEvent_t kev[2 * LPA_NBIN_KINDS];
-        for (auto &x : ev) HIPCHK(hipEventCreate(&x));
-        for (auto &x : kev) HIPCHK(hipEventCreate(&x));
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        // per launch (bins, then heavy sets): a family's time is the sum of
+        // its launches, wherever they fall in the step
+        const size_t nb = e->bins.size(), nl = nb + e->heavy.size();
+        std::vector<hipEvent_t> kev(2 * nl, nullptr);
+        std::vector<int> kind_of_launch(nl, -1);
+        for (size_t i = 0; i < nb; i++) {
+            const Bin &b = e->bins[i];
+            if (b.kind != LPA_BIN_SKIP && b.kind != LPA_BIN_CMS && b.kind != LPA_BIN_GLOBAL && b.end > b.begin)
+                kind_of_launch[i] = b.kind;
+        }
+        for (size_t j = 0; j < e->heavy.size(); j++)
+            if (e->heavy[j]->nh) kind_of_launch[nb + j] = e->heavy[j]->cms ? LPA_BIN_CMS : LPA_BIN_GLOBAL;
+        auto destroy = [&] {
+            for (auto &x : ev)
+                if (x) (void)hipEventDestroy(x);
+            for (auto &x : kev)
+                if (x) (void)hipEventDestroy(x);
+        };
         std::vector<double> kind_sum(LPA_NBIN_KINDS, 0.0);
         const bool had_graph_mode = e->graph_mode;
         try {
+            for (auto &x : ev) HIPCHK(hipEventCreate(&x));
+            for (auto &x : kev) HIPCHK(hipEventCreate(&x));
             if (method < 0 || method > 2) throw Fail(LPA_ERR_ARG, "method: 0 NOVA, 1 CGLP, 2 CSR");
+            if (use_graph && e->host_stream)
+                throw Fail(LPA_ERR_STATE, "the stream is host-resident: steps are not captured");
             if (method >= 1 && !e->have_csr) {
                 cglp_decode(e);
                 e->sync();
@@ -1922,7 +2211,7 @@ int lpa_time_iterations(lpa_engine *e, int method, int iters, int use_graph, flo
             for (int it = 0; it < iters; it++) {
                 HIPCHK(hipEventRecord(ev[0], e->stream));
                 if (e->graph_mode) enqueue_step(e, method);
-                else if (method == 0) e->step(false, kev);
+                else if (method == 0) e->step(false, kev.data());
                 else method_step(e, method);
                 HIPCHK(hipEventRecord(ev[1], e->stream));
                 HIPCHK(hipMemcpyAsync(e->h_counter, e->counter.p, 8, hipMemcpyDeviceToHost, e->stream));
@@ -1930,28 +2219,23 @@ int lpa_time_iterations(lpa_engine *e, int method, int iters, int use_graph, flo
                 HIPCHK(hipEventElapsedTime(&ms[it], ev[0], ev[1]));
                 if (changed) changed[it] = *e->h_counter;
                 e->cur ^= 1;
-                if (method == 0 && !e->graph_mode) {
-                    bool seen[LPA_NBIN_KINDS] = {false};
-                    for (const Bin &b : e->bins)
-                        if (b.kind != LPA_BIN_SKIP && !seen[b.kind] && b.end > b.begin) {
-                            seen[b.kind] = true;
-                            float t = 0;
-                            if (hipEventElapsedTime(&t, kev[2 * b.kind], kev[2 * b.kind + 1]) == hipSuccess)
-                                kind_sum[b.kind] += t;
-                        }
+                if (method == 0 && !e->graph_mode && !e->host_stream) {
+                    for (size_t l = 0; l < nl; l++) {
+                        if (kind_of_launch[l] < 0) continue;
+                        float t = 0;
+                        HIPCHK(hipEventElapsedTime(&t, kev[2 * l], kev[2 * l + 1]));
+                        kind_sum[kind_of_launch[l]] += t;
+                    }
                 }
             }
         } catch (...) {
             e->graph_mode = had_graph_mode;
-            for (auto &x : ev) (void)hipEventDestroy(x);
-            for (auto &x : kev) (void)hipEventDestroy(x);
+            destroy();
             throw;
         }
         e->graph_mode = had_graph_mode;
         for (int k = 0; k < LPA_NBIN_KINDS; k++) e->kind_ms[k] = (float)(kind_sum[k] / iters);
-        for (auto &x : ev) HIPCHK(hipEventDestroy(x));
-        for (auto &x : kev) HIPCHK(hipEventDestroy(x));
-        (void)use_graph;
+        destroy();
     });
 }
 

@@ -203,10 +203,24 @@ __global__ void __launch_bounds__(BLOCK) k_lane(StepArgs a) {
         if (a.rec) atomicAdd(&a.rec[sv[threadIdx.x]], cycles() - t0);
     }
     if (STAGE) {
-        // every row of a k_lane<N> bin holds at most N ids
+        // every row of a k_lane<N> bin holds at most N ids; each lane issues
+        // a round's 8 loads before its LDS stores
         uint64_t tot = a.row[first + nact] - b0;
         tot = tot < (uint64_t)N * BLOCK ? tot : (uint64_t)N * BLOCK;
-        for (uint32_t x = threadIdx.x; x < (uint32_t)tot; x += BLOCK) rows[x] = a.nbr[b0 + x];
+        constexpr int U = 8;
+        for (uint32_t x0 = 0; x0 < (uint32_t)tot; x0 += U * BLOCK) {
+            uint32_t r[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const uint32_t x = x0 + u * BLOCK + threadIdx.x;
+                r[u] = a.nbr[b0 + (x < (uint32_t)tot ? x : 0)];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const uint32_t x = x0 + u * BLOCK + threadIdx.x;
+                if (x < (uint32_t)tot) rows[x] = r[u];
+            }
+        }
     }
     __syncthreads();
     const uint32_t K = a.K, pairs = nact * K;
@@ -219,15 +233,29 @@ __global__ void __launch_bounds__(BLOCK) k_lane(StepArgs a) {
         const uint32_t d = active ? sd[vb] : 0;
         const uint64_t idx = (uint64_t)sv[vb] * K + k;
         const uint32_t old = active ? a.lin[idx] : UNL, own = clean_label(old);
+        // the neighbours' labels: a lane issues all its loads before using
+        // any (a slot beyond its row reads its own label word), and slots
+        // beyond every row of the wave are skipped by a wave-uniform test
+        uint32_t dw = d;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const uint32_t y = __shfl_xor(dw, o, 64);
+            dw = y > dw ? y : dw;
+        }
+        dw = __builtin_amdgcn_readfirstlane(dw);
+        const uint64_t rb = CSR ? (STAGE ? sbase[CSR ? vb : 0] - b0 : sbase[CSR ? vb : 0]) : 0;
         uint32_t x[N];
 #pragma unroll
         for (int j = 0; j < N; j++) {
-            uint32_t u = 0;
-            if ((uint32_t)j < d)
-                u = STAGE ? rows[(uint32_t)(sbase[CSR ? vb : 0] - b0) + j]
-                    : CSR ? a.nbr[sbase[CSR ? vb : 0] + j] : list[CSR ? 0 : j][vb];
-            x[j] = (uint32_t)j < d ? clean_label(a.lin[(uint64_t)u * K + k]) : UNL;
+            x[j] = UNL;
+            if ((uint32_t)j < dw) {
+                const uint32_t jj = (uint32_t)j < d ? (uint32_t)j : 0u;
+                const uint32_t u = STAGE ? rows[(uint32_t)rb + jj] : CSR ? a.nbr[rb + jj] : list[CSR ? 0 : j][vb];
+                x[j] = a.lin[(uint32_t)j < d ? (uint64_t)u * K + k : idx];
+            }
         }
+#pragma unroll
+        for (int j = 0; j < N; j++) x[j] = (uint32_t)j < d ? clean_label(x[j]) : UNL;
         // own label a strict majority of the labelled neighbours -> it is
         // the mode; otherwise sort and take the longest run
         uint32_t oc = 0, lc = 0;

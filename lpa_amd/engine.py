@@ -41,6 +41,8 @@ EXPORTS = {
     "lpa_cglp_csr_get": (C.c_int, [_VP, _VP, _VP, C.POINTER(C.c_uint64)]),
     "lpa_time_iterations": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, _VP, _VP]),
     "lpa_set_graph_mode": (C.c_int, [_VP, C.c_int]),
+    "lpa_stream_residence": (C.c_int, [_VP, C.c_int, C.c_uint64]),
+    "lpa_stream_batches": (C.c_int, [_VP, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "lpa_autotune": (C.c_int, [_VP, C.c_int, _VP, _VP]),
     "lpa_model_features": (C.c_int, [_VP, _VP]),
     "lpa_record_family": (C.c_int, [_VP, C.c_int, _VP, _VP]),
@@ -213,6 +215,18 @@ class Engine:
 
     def set_graph_mode(self, on=True):
         _check(self._L.lpa_set_graph_mode(self._h, int(on)))
+
+    def stream_residence(self, host, batch_bytes=0):
+        """keep the CGR stream in pinned host memory (host=True) and stream
+        it to the device in batches of at most batch_bytes every step (0:
+        1 GiB), or back in HBM (host=False)"""
+        _check(self._L.lpa_stream_residence(self._h, 1 if host else 0, int(batch_bytes)))
+
+    def stream_batches(self):
+        """(batches of the current layout, bytes of both staging buffers)"""
+        nb, sb = C.c_uint64(), C.c_uint64()
+        _check(self._L.lpa_stream_batches(self._h, C.byref(nb), C.byref(sb)))
+        return nb.value, sb.value
 
     def schedule_bins(self, bins):
         b = np.ascontiguousarray(bins, np.uint8)
