@@ -1,8 +1,9 @@
 """Text summaries of rocprofv3 output for the benchmark's headline phase.
 
 The benchmark's profiled command (bench.py --headline-only ...) builds the
-graph, trains the cost model, lays the graph out for the learned schedule
-(the last k_relayout dispatch) and then runs warmup + steps iterations.
+graph, lays it out (with --no-learned the degree schedule; otherwise the
+judged learned schedule), the last k_relayout dispatch, and then runs
+warmup + steps iterations.
 Everything dispatched after that last relayout belongs to those iterations,
 except the copy kernel of the calibration (bench.py --copy-calibration:
 k_calibration_copy over 256 MiB), whose FETCH_SIZE / WRITE_SIZE readings
@@ -114,7 +115,7 @@ def main():
     a = ap.parse_args()
     per, fam = trace_summary(a.trace, a.iters, a.steps)
     lines = [f"rocprofv3 kernel trace, headline phase of {a.workload} K={a.K}: the {a.steps} timed iterations "
-             f"(of {a.iters}) after the learned-schedule layout", "commands:"]
+             f"(of {a.iters}) after the last layout build", "commands:"]
     lines += [f"  {c.strip()}" for c in a.command.split(";") if c.strip()]
     lines.append("")
     lines.append(f"{'kernel':78s} {'calls':>6s} {'avg_us':>9s} {'us/iter':>9s} family")
