@@ -229,16 +229,20 @@ template <class S> LPA_HD void encode_record(S &s, uint32_t v, const uint32_t *a
 
 }  // namespace enc
 
-// Record length of every compressed position (vid[p] names its vertex);
-// *err = 1 + vertex of the first row that is not strictly increasing or has
-// an id >= n (the format's gaps need sorted, duplicate-free rows).
+// Record length of every compressed position (vid[p] names its vertex)
+// whose row has at most long_min ids (longer rows are measured by the
+// workgroup kernels of encode_long.hpp); *err = 1 + vertex of the first row
+// that is not strictly increasing or has an id >= n (the format's gaps need
+// sorted, duplicate-free rows).
 __global__ void __launch_bounds__(256) k_encode_len(const uint64_t *__restrict__ row, const uint32_t *__restrict__ col,
                                                     const uint32_t *__restrict__ vid, uint64_t n_c, uint32_t n,
-                                                    uint64_t *__restrict__ len, unsigned long long *__restrict__ err) {
+                                                    uint64_t *__restrict__ len, unsigned long long *__restrict__ err,
+                                                    uint32_t long_min) {
     const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n_c) return;
     const uint32_t v = vid[p];
     const uint64_t r0 = row[v], r1 = row[v + 1];
+    if (r1 >= r0 && r1 - r0 > long_min) return;
     const uint32_t d = (uint32_t)(r1 - r0);
     const uint32_t *adj = col + r0;
     bool bad = r1 < r0;
@@ -255,11 +259,13 @@ __global__ void __launch_bounds__(256) k_encode_len(const uint64_t *__restrict__
 
 __global__ void __launch_bounds__(256) k_encode_write(const uint64_t *__restrict__ row, const uint32_t *__restrict__ col,
                                                       const uint32_t *__restrict__ vid, uint64_t n_c,
-                                                      const uint64_t *__restrict__ offs, uint64_t *words) {
+                                                      const uint64_t *__restrict__ offs, uint64_t *words,
+                                                      uint32_t long_min) {
     const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n_c) return;
     const uint32_t v = vid[p];
     const uint64_t r0 = row[v];
+    if (row[v + 1] - r0 > long_min) return;
     const uint32_t d = (uint32_t)(row[v + 1] - r0);
     enc::BitWriter<enc::DeviceOr> w(words, offs[p], offs[p + 1]);
     enc::encode_record(w, v, col + r0, d);

@@ -26,6 +26,7 @@
 
 #include "bfs_relabel.hpp"
 #include "encode_kernels.hpp"
+#include "encode_long.hpp"
 #include "heavy_kernels.hpp"
 #include "mlp_kernels.hpp"
 #include "lp_kernels.hpp"
@@ -885,7 +886,33 @@ int lpa_graph_encode_csr(lpa_engine *e, uint64_t n, const uint64_t *row, const u
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, e->stream, e->counter.p, (uint64_t)1, 0ull);
         if (n_c)
             hipLaunchKernelGGL(k_encode_len, dim3((unsigned)((n_c + 255) / 256)), dim3(256), 0, e->stream, drow.p,
-                               dcol.p, e->vid0.p, n_c, (uint32_t)n, len.p, e->counter.p);
+                               dcol.p, e->vid0.p, n_c, (uint32_t)n, len.p, e->counter.p, enc::LONG_ROW);
+        // rows longer than LONG_ROW: one workgroup each, with scratch of
+        // d + 1 entries per row for every per-entry array
+        std::vector<uint32_t> lpos;
+        std::vector<uint64_t> lbase(1, 0);
+        for (uint64_t p = 0; p < n_c; p++)
+            if (deg(vert[p]) > enc::LONG_ROW) {
+                lpos.push_back((uint32_t)p);
+                lbase.push_back(lbase.back() + deg(vert[p]) + 1);
+            }
+        DevBuf<uint32_t> dlpos, lleft, llen, lrid, lnxt_i, lnxt_r, lseg_i, lseg_r;
+        DevBuf<uint64_t> dlbase, lP_i, lP_r;
+        DevBuf<enc::LongInfo> linfo;
+        enc::LongScratch lsc{};
+        if (!lpos.empty()) {
+            const uint64_t tot = lbase.back();
+            dlpos.upload(lpos.data(), lpos.size(), e->stream);
+            dlbase.upload(lbase.data(), lbase.size(), e->stream);
+            for (DevBuf<uint32_t> *b : {&lleft, &llen, &lrid, &lnxt_i, &lnxt_r, &lseg_i, &lseg_r}) b->alloc(tot);
+            lP_i.alloc(tot);
+            lP_r.alloc(tot);
+            linfo.alloc(lpos.size());
+            lsc = enc::LongScratch{dlbase.p, lleft.p, llen.p, lrid.p, lnxt_i.p, lnxt_r.p, lseg_i.p, lseg_r.p,
+                                   lP_i.p, lP_r.p, linfo.p};
+            hipLaunchKernelGGL(k_encode_long_len, dim3((unsigned)lpos.size()), dim3(enc::LONG_LANES), 0, e->stream,
+                               drow.p, dcol.p, e->vid0.p, dlpos.p, (uint32_t)n, lsc, len.p, e->counter.p);
+        }
         const uint64_t nb = std::max<uint64_t>(1, (n_c + SCAN_PER_BLOCK - 1) / SCAN_PER_BLOCK);
         DevBuf<uint64_t> sums;
         sums.alloc(nb);
@@ -908,7 +935,10 @@ int lpa_graph_encode_csr(lpa_engine *e, uint64_t n, const uint64_t *row, const u
                            (unsigned long long *)e->words0.p, n64, 0ull);
         if (n_c)
             hipLaunchKernelGGL(k_encode_write, dim3((unsigned)((n_c + 255) / 256)), dim3(256), 0, e->stream, drow.p,
-                               dcol.p, e->vid0.p, n_c, e->offs0.p, e->words0.p);
+                               dcol.p, e->vid0.p, n_c, e->offs0.p, e->words0.p, enc::LONG_ROW);
+        if (!lpos.empty())
+            hipLaunchKernelGGL(k_encode_long_write, dim3((unsigned)lpos.size()), dim3(enc::LONG_LANES), 0, e->stream,
+                               e->vid0.p, dlpos.p, lsc, e->offs0.p, e->words0.p);
         HIPCHK(hipGetLastError());
         if (e->host_stream) move_stream(e, e->words0, true);
         finish_graph(e);
@@ -938,7 +968,7 @@ int lpa_encode_csr_device(lpa_engine *e, uint64_t n, uint64_t n_c, const uint64_
         }
         // record lengths into offsets[0..n_c), then an in-place exclusive scan
         hipLaunchKernelGGL(k_encode_len, dim3((unsigned)((n_c + 255) / 256)), dim3(256), 0, e->stream, row, col, vid,
-                           n_c, (uint32_t)n, offsets, (unsigned long long *)scr);
+                           n_c, (uint32_t)n, offsets, (unsigned long long *)scr, UINT32_MAX);
         const uint64_t nb = (n_c + SCAN_PER_BLOCK - 1) / SCAN_PER_BLOCK;
         hipLaunchKernelGGL(k_scan_sums, dim3((unsigned)nb), dim3(256), 0, e->stream, offsets, n_c, scr + 1);
         hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, e->stream, scr + 1, nb);
@@ -961,7 +991,7 @@ int lpa_encode_csr_device(lpa_engine *e, uint64_t n, uint64_t n_c, const uint64_
             hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, e->stream,
                                (unsigned long long *)words, nw, 0ull);
         hipLaunchKernelGGL(k_encode_write, dim3((unsigned)((n_c + 255) / 256)), dim3(256), 0, e->stream, row, col, vid,
-                           n_c, offsets, words);
+                           n_c, offsets, words, UINT32_MAX);
         HIPCHK(hipGetLastError());
         e->sync();
         *nbits = total;

@@ -1,11 +1,14 @@
 // decode_check.hip -- test library: runs the device CGR decoders of
-// lpa_amd/csrc/hip/cgr_decode.hpp and the device record encoder of
-// encode_kernels.hpp on the host, so the tests can compare them with the
-// CPU oracle without a GPU.
+// lpa_amd/csrc/hip/cgr_decode.hpp, the device record encoder of
+// encode_kernels.hpp and the phases of the long-row encoder of
+// encode_long.hpp (lane by lane) on the host, so the tests can compare them
+// with the CPU oracle and with each other without a GPU.
 #include <cstdint>
+#include <vector>
 
 #include "../../lpa_amd/csrc/hip/cgr_decode.hpp"
 #include "../../lpa_amd/csrc/hip/encode_kernels.hpp"
+#include "../../lpa_amd/csrc/hip/encode_long.hpp"
 
 namespace {
 struct HostOr {   // one thread writes the whole stream here
@@ -75,6 +78,52 @@ void dc_encode_record(uint32_t v, const uint32_t *adj, uint32_t d, uint64_t *wor
     lpa::enc::BitWriter<HostOr> w(words, start, end);
     lpa::enc::encode_record(w, v, adj, d);
     w.finish();
+}
+
+// The long-row encoder's phases for one row, run for lanes 0..T-1 in turn
+// between the barriers of k_encode_long_len / k_encode_long_write.  Returns
+// the record's bits (UINT64_MAX for an unsorted row or an id >= n) and, when
+// words is not null, ORs the record into it at bit `start`.
+uint64_t dc_encode_long(uint32_t v, const uint32_t *adj, uint32_t d, uint32_t n, uint32_t T, uint64_t *words,
+                        uint64_t start) {
+    using namespace lpa::enc;
+    std::vector<uint64_t> ni(T), nr(T), oi(T), orr(T);
+    bool ok = true;
+    for (uint32_t t = 0; t < T; t++) ok = runs_count(adj, d, n, t, T, ni[t], nr[t]) && ok;
+    if (!ok) return UINT64_MAX;
+    uint64_t NI = 0, NR = 0;
+    for (uint32_t t = 0; t < T; t++) {
+        oi[t] = NI;
+        orr[t] = NR;
+        NI += ni[t];
+        NR += nr[t];
+    }
+    std::vector<uint32_t> left(d + 1), len(d + 1), rid(d + 1), nxt_i(d + 1), nxt_r(d + 1), seg_i(d + 1),
+        seg_r(d + 1);
+    std::vector<uint64_t> P_i(d + 1), P_r(d + 1);
+    for (uint32_t t = 0; t < T; t++) runs_place(adj, d, t, T, oi[t], orr[t], left.data(), len.data(), rid.data());
+    LongBlock bi{true, v, left.data(), len.data(), NI, P_i.data(), nxt_i.data(), seg_i.data()};
+    LongBlock br{false, v, rid.data(), nullptr, NR, P_r.data(), nxt_r.data(), seg_r.data()};
+    uint64_t segs[2], bits[2];
+    const LongBlock *blocks[2] = {&bi, &br};
+    for (int k = 0; k < 2; k++) {
+        const LongBlock &B = *blocks[k];
+        std::vector<uint64_t> sums(T);
+        for (uint32_t t = 0; t < T; t++) sums[t] = later_sum(B, t, T);
+        uint64_t base = 0;
+        for (uint32_t t = 0; t < T; t++) {
+            later_prefix(B, t, T, base);
+            base += sums[t];
+        }
+        for (uint32_t t = 0; t < T; t++) segment_ends(B, t, T);
+        segs[k] = segment_chain(B);
+        bits[k] = block_bits(B, segs[k]);
+    }
+    if (words) {
+        for (uint32_t t = 0; t < T; t++) write_segments<HostOr>(bi, segs[0], start, words, t, T);
+        for (uint32_t t = 0; t < T; t++) write_segments<HostOr>(br, segs[1], start + bits[0], words, t, T);
+    }
+    return bits[0] + bits[1];
 }
 
 }  // extern "C"

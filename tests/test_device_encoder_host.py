@@ -68,3 +68,78 @@ def test_generated_vs_host_encoder(dc, gen):
     assert np.array_equal(starts, cgr.offsets)
     host = np.frombuffer(cgr.stream_bytes(), np.uint8)
     assert np.array_equal(stream, host[:len(stream)])
+
+
+# --------------------------------------------------------------- long rows
+
+def _long_row(rng, n, d_target):
+    """sorted distinct ids: runs of 1-3 ids (residual entries), 4-20, and a
+    few of hundreds to thousands (interval entries), separated by gaps of
+    2 to 10,000"""
+    out, x = [], int(rng.integers(0, 1000))
+    while len(out) < d_target:
+        r = rng.random()
+        run = (1 if r < 0.55 else 2 if r < 0.65 else 3 if r < 0.7 else
+               int(rng.integers(4, 21)) if r < 0.97 else int(rng.integers(100, 5000)))
+        gap = int(rng.integers(2, 10000)) if rng.random() < 0.9 else 2
+        if x + run >= n:
+            break
+        out.extend(range(x, x + run))
+        x += run + gap
+    return np.array(out, np.uint32)
+
+
+@pytest.fixture(scope="module")
+def dcl(dc):
+    dc.dc_encode_long.restype = C.c_uint64
+    dc.dc_encode_long.argtypes = [C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+                                  C.c_uint64]
+    return dc
+
+
+def _both(dc, v, adj, n, T, start):
+    d = len(adj)
+    adj = np.ascontiguousarray(adj, np.uint32) if d else np.zeros(1, np.uint32)
+    bits = dc.dc_encode_bits(v, adj.ctypes.data, d)
+    nw = (start + bits) // 64 + 4
+    w1 = np.zeros(nw, np.uint64)
+    w2 = np.zeros(nw, np.uint64)
+    dc.dc_encode_record(v, adj.ctypes.data, d, w1.ctypes.data, start, start + bits)
+    lbits = dc.dc_encode_long(v, adj.ctypes.data, d, n, T, w2.ctypes.data, start)
+    return bits, lbits, w1, w2
+
+
+@pytest.mark.parametrize("T", [1, 7, 64, 1024])
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_long_row_phases_match_record_encoder(dcl, T, seed):
+    """the workgroup phases (lane by lane) give the record encoder's bits
+    and length, at unaligned record starts, with ids up to 2^30"""
+    rng = np.random.default_rng(seed)
+    n = (1 << 30) - 1
+    for d_target, start in ((70000, 13), (3000, 64 * 5 + 37), (200000, 0)):
+        adj = _long_row(rng, n, d_target)
+        v = int(rng.integers(0, n))
+        bits, lbits, w1, w2 = _both(dcl, v, adj, n, T, start)
+        assert lbits == bits, (d_target, T)
+        assert np.array_equal(w1, w2), (d_target, T)
+
+
+@pytest.mark.parametrize("case", ["empty", "one", "one_run", "singles", "runs_of_3_and_4", "top_ids"])
+def test_long_row_edge_cases(dcl, case):
+    n = 1 << 30
+    adj = {"empty": np.zeros(0, np.uint32), "one": np.array([5], np.uint32),
+           "one_run": np.arange(100, 20100, dtype=np.uint32),
+           "singles": np.arange(0, 400000, 2, dtype=np.uint32),
+           "runs_of_3_and_4": np.concatenate([np.arange(10 * k, 10 * k + (3 + k % 2)) for k in range(5000)]).astype(np.uint32),
+           "top_ids": np.arange(n - 3000, n, 3, dtype=np.uint32)}[case]
+    for T in (1, 64, 1024):
+        for v in (0, 7, n - 1):
+            bits, lbits, w1, w2 = _both(dcl, v, adj, n, T, 29)
+            assert lbits == bits and np.array_equal(w1, w2), (case, T, v)
+
+
+def test_long_row_rejects_bad_rows(dcl):
+    n = 1000
+    for adj in (np.array([3, 2, 5], np.uint32), np.array([3, 3, 5], np.uint32), np.array([1, 2, 1000], np.uint32)):
+        for T in (1, 2, 1024):
+            assert dcl.dc_encode_long(0, adj.ctypes.data, len(adj), n, T, None, 0) == 2**64 - 1

@@ -39,7 +39,29 @@ def test_golden_cases(case):
     e.close()
 
 
-@pytest.mark.parametrize("gen,T", [("rmat", 0), ("lattice", 0), ("copying", 32), ("hubs", 0)])
+def _long_rows_graph(seed=25):
+    """background edges plus five hubs whose rows are longer than the
+    per-lane encoder's limit (65,536 ids): mixes of short runs, intervals
+    and runs of thousands of ids, so the workgroup kernels encode them"""
+    rng = np.random.default_rng(seed)
+    n = 2_000_000
+    src = [rng.integers(0, n, 1_000_000)]
+    dst = [rng.integers(0, n, 1_000_000)]
+    for h, (d, runmax) in enumerate(((70_000, 8), (150_000, 40), (300_000, 3), (90_000, 5000), (66_000, 1))):
+        ids, x = [], int(rng.integers(10, 100))
+        while len(ids) < d and x < n - runmax:
+            run = int(rng.integers(1, runmax + 1))
+            ids.extend(range(x, x + run))
+            x += run + int(rng.integers(2, 30))
+        ids = np.array(ids[:d], np.int64)
+        src.append(np.full(len(ids), h + 1))
+        dst.append(ids)
+    src = np.concatenate(src).astype(np.uint32)
+    dst = np.concatenate(dst).astype(np.uint32)
+    return L.Csr.from_edges(n, src, dst, False)
+
+
+@pytest.mark.parametrize("gen,T", [("rmat", 0), ("lattice", 0), ("copying", 32), ("hubs", 0), ("long_rows", 0)])
 def test_generated_graphs(gen, T):
     if gen == "rmat":
         g = L.gen_rmat(16, 16, seed=21)
@@ -47,6 +69,9 @@ def test_generated_graphs(gen, T):
         g = L.gen_lattice(512, 0.1, 0.005, seed=22)
     elif gen == "copying":
         g = L.gen_copying(100000, 30.0, 2.1, 20000, 0.7, 500, seed=23)
+    elif gen == "long_rows":
+        g = _long_rows_graph()
+        assert (g.degrees() > 65536).sum() == 5
     else:
         rng = np.random.default_rng(24)
         n = 300000
