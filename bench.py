@@ -156,8 +156,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-cglp", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
-    ap.add_argument("--device-encode", action="store_true",
-                    help="also time CGR-encoding the workload's CSR on the device")
+    ap.add_argument("--no-device-encode", action="store_true",
+                    help="skip timing the CGR encoding of the workload's CSR on the device")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the headline phase (for profiling): no K=1, two-kernel or degree-binned K runs")
     ap.add_argument("--stream-host", type=int, default=0, metavar="BATCH_BYTES",
@@ -179,7 +179,7 @@ def main():
     m = int(g.m)
     log(f"graph n={g.n} m={m} bits={cgr.nbits} built in {time.time() - t0:.1f}s (host CGR encode {host_enc:.2f}s)")
     dev_enc = None
-    if args.device_encode:
+    if not args.no_device_encode and not args.headline_only:
         dev_enc = device_encode_seconds(L, g, w)
         log(f"device CGR encode {dev_enc:.3f}s")
 

@@ -5,8 +5,9 @@ with K = 8 (count-min sketch 4 x 1024 above degree 4096), bit for bit
 against the oracle's own trajectory after every step.
 
 R-MAT scale 27 (134 M vertices, 8.36 G arcs, K = 8, count-min sketch 4 x 2048
-above degree 8192; ten steps) and R-MAT scale 24 (16.8 M vertices, K = 64
-semi-supervised vectors with 1 % seeds from 100 classes; fifteen steps) are
+above degree 8192; ten steps), R-MAT scale 24 (16.8 M vertices, K = 64
+semi-supervised vectors with 1 % seeds from 100 classes; fifteen steps) and
+the copying model at 50 M vertices (1.5 G arcs, T = 32; six steps) are
 too large for the oracle to follow a whole trajectory in minutes, so they
 take one-step checks: after a step the oracle recomputes L_{t+1} from the
 engine's own L_t.  Every vertex and every vector is recomputed after the
@@ -135,3 +136,16 @@ def test_rmat24_k64_semi_one_step_checks():
         idx = rng.choice(n, n // 100, replace=False)
         lab[idx, k] = rng.integers(0, 100, len(idx), dtype=np.uint32)
     one_step_checks(cgr, 15, (0, 14), K, semi=True, labels0=lab.reshape(-1), sample=20_000)
+
+
+def test_copy50m_one_step_checks():
+    """the copying model at 50 M vertices (mean out-degree 30, power law 2.1
+    capped at 100 k, 70 % copied within +-1000, seed 2; directed), T = 32 (a
+    small-vertex CSR part), K = 1, CMS 4 x 2048 above degree 8192; six steps,
+    every vertex recomputed after the first and the last"""
+    t0 = time.perf_counter()
+    g = L.gen_copying(50_000_000, 30.0, 2.1, 100000, 0.7, 1000, seed=2)
+    cgr = g.encode(32, 128)
+    log(f"copying 50M: n={g.n} m={g.m} n_c={cgr.n_c} built in {time.perf_counter() - t0:.0f} s")
+    del g
+    one_step_checks(cgr, 6, (0, 5), 1, cms_thresh=8192, cms_depth=4, cms_width=2048, cms_seed=0)
