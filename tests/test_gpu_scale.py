@@ -39,8 +39,25 @@ def checksum(g):
     return h.hexdigest()
 
 
+_capsys = None
+
+
+@pytest.fixture(autouse=True)
+def _live_log(capsys):
+    """progress lines bypass pytest's capture: the full-size tests run for
+    minutes, and a run that prints nothing for that long looks hung"""
+    global _capsys
+    _capsys = capsys
+    yield
+    _capsys = None
+
+
 def log(msg):
-    print(f"[scale] {msg}", flush=True)
+    if _capsys is None:
+        print(f"[scale] {msg}", flush=True)
+        return
+    with _capsys.disabled():
+        print(f"[scale] {msg}", flush=True)
 
 
 @pytest.fixture(scope="module")
@@ -111,6 +128,7 @@ def test_rmat27_k8_one_step_checks():
     """R-MAT scale 27, edge factor 32, seed 4, BFS order; K = 8 from L0 = v;
     CMS 4 x 2048 above degree 8192; ten steps"""
     t0 = time.perf_counter()
+    log("R-MAT 27: generating (host draws, device BFS relabelling)")
     g = L.gen_rmat_device_bfs(27, 32, 0.57, 0.19, 0.19, seed=4)
     log(f"R-MAT 27: n={g.n} m={g.m} (BFS relabelled on the device) in {time.perf_counter() - t0:.0f} s; "
         f"checksum {checksum(g)}")
@@ -125,7 +143,9 @@ def test_rmat24_k64_semi_one_step_checks():
     """R-MAT scale 24, edge factor 16, seed 3, BFS order; K = 64 vectors, each
     with 1 % of the vertices seeded from 100 classes (clamped), the rest
     unlabelled; exact mode; fifteen steps"""
+    log("R-MAT 24: generating (host draws, device BFS relabelling)")
     g = L.gen_rmat_device_bfs(24, 16, 0.57, 0.19, 0.19, seed=3)
+    log(f"R-MAT 24: n={g.n} m={g.m}")
     assert checksum(g) == "d50d1452d707001b"   # the host-relabelled graph's (test_generators.py)
     cgr = g.encode(0, 128)
     n, K = g.n, 64
@@ -144,6 +164,7 @@ def test_copy50m_one_step_checks():
     small-vertex CSR part), K = 1, CMS 4 x 2048 above degree 8192; six steps,
     every vertex recomputed after the first and the last"""
     t0 = time.perf_counter()
+    log("copying 50M: generating")
     g = L.gen_copying(50_000_000, 30.0, 2.1, 100000, 0.7, 1000, seed=2)
     cgr = g.encode(32, 128)
     log(f"copying 50M: n={g.n} m={g.m} n_c={cgr.n_c} built in {time.perf_counter() - t0:.0f} s")
