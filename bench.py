@@ -255,6 +255,17 @@ def main():
     if args.copy_calibration:
         cms_ = e.calibration_copy(256 << 20, 3)
         log(f"copy calibration: 3 copies of 256 MiB, {cms_:.3f} ms each ({2 * 256 * 1.048576 / cms_:.0f} GB/s)")
+    gather = None
+    if not args.stream_host and not args.copy_calibration:
+        # the ceiling of the step's dominant access: random reads of label
+        # rows (the K labels of a vertex; 8 at a time beyond K = 8) from a
+        # buffer the size of a label array, one per arc
+        words = 1 if K == 1 else 2 if K < 4 else 4 if K < 8 else 8
+        g_ms = e.calibration_gather(4 * K * g.n, words, m, 3)
+        gather = {"row_bytes": 4 * words, "footprint_bytes": 4 * K * int(g.n), "gathers": m,
+                  "ms": g_ms, "rows_per_s": m / (g_ms / 1e3)}
+        log(f"random-gather ceiling: {m} reads of {4 * words} B rows over {4 * K * g.n / 1e9:.2f} GB: "
+            f"{g_ms:.2f} ms ({gather['rows_per_s'] / 1e9:.1f} G rows/s)")
     stats = e.stats()
     ms_per_step = float(np.mean(head_ms))
     value = K * m * args.steps / (float(np.sum(head_ms)) / 1e3)
@@ -290,6 +301,7 @@ def main():
                                "frac": it_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS},
         "kernel_ms_per_step": head_kt,
         "bins": {k: v for k, v in stats.items() if v["vertices"]},
+        "gather_ceiling": gather,
         "device": e.info(),
         "encode": {"host_s": host_enc, "host_threads": L.threads(), "device_s": dev_enc,
                    "note": "CGR encode of the workload's CSR; device time includes the CSR upload"},
@@ -303,6 +315,18 @@ def main():
     if tr:
         line["roofline"]["traffic"] = tr["bytes"]
         line["roofline"]["traffic_source"] = tr["source"]
+    if gather:
+        # arcs per second of the step and of the dominant family against the
+        # random-gather ceiling (each arc gathers one row per 8 vectors)
+        per_arc = max(1, (K + 7) // 8)
+        step_rows = m * per_arc / (ms_per_step / 1e3)
+        gather["step_rows_per_s"] = step_rows
+        gather["step_frac"] = step_rows / gather["rows_per_s"]
+        if dom in stats and dom_ms > 0:
+            fam_rows = stats[dom]["arcs"] * per_arc / (dom_ms / 1e3)
+            gather["family"] = dom
+            gather["family_rows_per_s"] = fam_rows
+            gather["family_frac"] = fam_rows / gather["rows_per_s"]
     if "degree_k1" in res:
         line["speedup_vs_degree_k1"] = value / res["degree_k1"]["edges_per_s"]
     if not args.no_cpu_baseline:

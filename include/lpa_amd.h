@@ -301,6 +301,15 @@ int lpa_last_kernel_times(lpa_engine *e, float *ms_per_kind /* LPA_NBIN_KINDS */
  * counters of the step kernels can be scaled by what they read for a copy
  * of known size; *ms = average kernel time. */
 int lpa_calibration_copy(lpa_engine *e, uint64_t bytes, int reps, float *ms);
+/* Measurement helper, no counterpart in the reference: random row gathers,
+ * the access pattern of a step's label reads.  `gathers` reads of rows of
+ * row_words 32-bit words (1, 2, 4, 8 or 16) at uniformly random rows of a
+ * fresh device buffer of `bytes` (k_calibration_gather, every read
+ * independent of the others), `reps` times; *ms = average kernel time, so
+ * gathers / ms is the rate the memory system sustains for this pattern at
+ * this footprint. */
+int lpa_calibration_gather(lpa_engine *e, uint64_t bytes, uint32_t row_words, uint64_t gathers, int reps,
+                           float *ms);
 
 #ifdef __cplusplus
 }

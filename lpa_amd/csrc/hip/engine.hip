@@ -2370,4 +2370,49 @@ int lpa_calibration_copy(lpa_engine *e, uint64_t bytes, int reps, float *ms) {
     });
 }
 
+int lpa_calibration_gather(lpa_engine *e, uint64_t bytes, uint32_t row_words, uint64_t gathers, int reps,
+                           float *ms) {
+    return guard([&] {
+        if (!e || !ms) throw Fail(LPA_ERR_ARG, "null argument");
+        if (row_words != 1 && row_words != 2 && row_words != 4 && row_words != 8 && row_words != 16)
+            throw Fail(LPA_ERR_RANGE, "row_words must be 1, 2, 4, 8 or 16");
+        const uint64_t nrows = bytes / (4ull * row_words);
+        if (nrows == 0 || gathers == 0 || reps <= 0) throw Fail(LPA_ERR_RANGE, "empty gather");
+        HIPCHK(hipSetDevice(e->device));
+        constexpr uint32_t PER_LANE = 64;
+        const uint64_t lanes = (gathers + PER_LANE - 1) / PER_LANE;
+        const uint64_t blocks = (lanes + 255) / 256;
+        if (blocks > 0x7FFFFFFFull) throw Fail(LPA_ERR_RANGE, "too many gathers");
+        DevBuf<uint32_t> src, out;
+        src.alloc(nrows * row_words);
+        out.alloc(blocks * 256);
+        HIPCHK(hipMemsetAsync(src.p, 1, nrows * row_words * 4, e->stream));
+        auto launch = [&](uint64_t seed) {
+            const dim3 g((unsigned)blocks), b(256);
+            switch (row_words) {
+            case 1: hipLaunchKernelGGL(k_calibration_gather<1>, g, b, 0, e->stream, src.p, nrows, PER_LANE, seed, out.p); break;
+            case 2: hipLaunchKernelGGL(k_calibration_gather<2>, g, b, 0, e->stream, src.p, nrows, PER_LANE, seed, out.p); break;
+            case 4: hipLaunchKernelGGL(k_calibration_gather<4>, g, b, 0, e->stream, src.p, nrows, PER_LANE, seed, out.p); break;
+            case 8: hipLaunchKernelGGL(k_calibration_gather<8>, g, b, 0, e->stream, src.p, nrows, PER_LANE, seed, out.p); break;
+            default: hipLaunchKernelGGL(k_calibration_gather<16>, g, b, 0, e->stream, src.p, nrows, PER_LANE, seed, out.p); break;
+            }
+        };
+        launch(12345);   // warm-up
+        hipEvent_t a = nullptr, b = nullptr;
+        HIPCHK(hipEventCreate(&a));
+        HIPCHK(hipEventCreate(&b));
+        HIPCHK(hipEventRecord(a, e->stream));
+        for (int r = 0; r < reps; r++) launch((uint64_t)r * 0x51ED27ull);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(b, e->stream));
+        HIPCHK(hipEventSynchronize(b));
+        float t = 0;
+        HIPCHK(hipEventElapsedTime(&t, a, b));
+        *ms = t / reps * (float)((double)gathers / (double)(lanes * PER_LANE));
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        e->sync();
+    });
+}
+
 }  // extern "C"

@@ -182,6 +182,39 @@ __global__ void __launch_bounds__(256) k_calibration_copy(const uint4 *__restric
     if (i < n) dst[i] = src[i];
 }
 
+// random row gathers (measurement): lane i reads `per_lane` rows of
+// W 32-bit words at uniformly random row indices of a buffer of nrows rows
+// (indices from a counter-based splitmix64 stream, so no load depends on
+// another) and stores one folded word, so the loads cannot be dropped; the
+// pattern of the label-row gathers of a step, at a rate the kernels can be
+// compared with
+template <int W>
+__global__ void __launch_bounds__(256) k_calibration_gather(const uint32_t *__restrict__ src, uint64_t nrows,
+                                                            uint32_t per_lane, uint64_t seed,
+                                                            uint32_t *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t acc = 0;
+#pragma unroll 8
+    for (uint32_t k = 0; k < per_lane; k++) {
+        uint64_t z = seed + (i * per_lane + k + 1) * 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        const uint32_t *row = src + (z % nrows) * W;
+        if (W % 4 == 0) {
+#pragma unroll
+            for (int q = 0; q < W / 4; q++) {
+                const uint4 w = reinterpret_cast<const uint4 *>(row)[q];
+                acc ^= w.x ^ w.y ^ w.z ^ w.w;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < W; q++) acc ^= row[q];
+        }
+    }
+    out[i] = acc;
+}
+
 __global__ void k_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;

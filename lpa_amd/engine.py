@@ -58,6 +58,7 @@ EXPORTS = {
                                               C.POINTER(C.c_int)]),
     "lpa_last_kernel_times": (C.c_int, [_VP, _VP]),
     "lpa_calibration_copy": (C.c_int, [_VP, C.c_uint64, C.c_int, _VP]),
+    "lpa_calibration_gather": (C.c_int, [_VP, C.c_uint64, C.c_uint32, C.c_uint64, C.c_int, _VP]),
     "lpa_graph_encode_csr": (C.c_int, [_VP, C.c_uint64, _VP, _VP, C.c_uint32]),
     "lpa_graph_stream": (C.c_int, [_VP, _VP, _VP, _VP, _VP, _VP]),
     "lpa_encode_csr_device": (C.c_int, [_VP, C.c_uint64, C.c_uint64, _VP, _VP, _VP, _VP, _VP, C.c_uint64,
@@ -325,6 +326,14 @@ class Engine:
         """plain device copy of nbytes, reps times; returns ms per copy"""
         ms = C.c_float(0)
         _check(self._L.lpa_calibration_copy(self._h, nbytes, reps, C.byref(ms)))
+        return ms.value
+
+    def calibration_gather(self, nbytes, row_words, gathers, reps=3):
+        """random gathers of rows of row_words 32-bit words from a buffer of
+        nbytes; returns ms per `gathers` reads"""
+        ms = C.c_float(0)
+        _check(self._L.lpa_calibration_gather(self._h, int(nbytes), int(row_words), int(gathers), reps,
+                                              C.byref(ms)))
         return ms.value
 
 
