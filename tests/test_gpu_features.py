@@ -57,3 +57,16 @@ def test_features_and_model_inputs(graph):
     lab1 = e.labels()
     assert np.array_equal(x[:, 7], (lab1 != np.arange(g.n, dtype=np.uint32)).astype(np.float32))
     e.close()
+
+
+def test_gather_calibration():
+    """the random-gather measurement runs for every row width and rejects
+    bad arguments"""
+    e = L.Engine(0)
+    for words in (1, 2, 4, 8, 16):
+        ms = e.calibration_gather(1 << 24, words, 1 << 20, 2)
+        assert 0 < ms < 1000
+    for bad in ((1 << 24, 3, 1 << 20), (0, 8, 1 << 20), (1 << 24, 8, 0)):
+        with pytest.raises(L.LpaError):
+            e.calibration_gather(*bad)
+    e.close()
