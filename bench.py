@@ -40,8 +40,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
-    # R-MAT scale 22, ef 16, seed 1; CMS 4x1024 for deg > 4096 (default)
-    "rmat22": dict(gen="rmat", scale=22, ef=16, seed=1, T=0, K=8,
+    # R-MAT scale 22, ef 16, seed 1; CMS 4x1024 for deg > 4096; K = 8 steps
+    # replayed from a captured HIP graph
+    "rmat22": dict(gen="rmat", scale=22, ef=16, seed=1, T=0, K=8, graph=True,
                    cms=dict(cms_thresh=4096, cms_depth=4, cms_width=1024, cms_seed=0)),
     # 1024x1024 lattice, 10% edges dropped, 0.5% shortcuts, seed 0; exact
     "lattice1024": dict(gen="lattice", side=1024, seed=0, T=0, K=1, cms=dict()),
@@ -54,8 +55,9 @@ WORKLOADS = {
     # a captured HIP graph
     "rmat24k64": dict(gen="rmat", scale=24, ef=16, seed=3, T=0, K=64, semi=True, cms=dict(), graph=True,
                       autotune=True),
-    # R-MAT scale 27, ef 32, seed 4; K = 8; CMS 4x2048 for deg > 8192
-    "rmat27": dict(gen="rmat", scale=27, ef=32, seed=4, T=0, K=8,
+    # R-MAT scale 27, ef 32, seed 4; K = 8; CMS 4x2048 for deg > 8192; steps
+    # replayed from a captured HIP graph (one launch per iteration)
+    "rmat27": dict(gen="rmat", scale=27, ef=32, seed=4, T=0, K=8, graph=True,
                    cms=dict(cms_thresh=8192, cms_depth=4, cms_width=2048, cms_seed=0)),
 }
 
