@@ -134,7 +134,8 @@ CASES = {
 
 def main():
     # the reference compressor is built here, from its sources under
-    # /root/reference, into oracle/_ref (git- and gpurun-ignored)
+    # /root/reference, into oracle/_ref (ignored by git and never shipped
+    # with the tree)
     subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
     if not os.path.exists(REF_BIN):
         sys.exit("the reference compressor did not build (oracle/Makefile `ref`)")
