@@ -17,8 +17,13 @@ def test_mlp_forward_matches_reference():
     m.params = p
     x = rng.normal(0, 2, (5000, 8)).astype(np.float32)
     y = m.forward(x)
-    want, _ = R.forward(p.astype(np.float64), x.astype(np.float64))
-    assert np.allclose(y, want, rtol=1e-4, atol=1e-4)
+    want, (_, _, _, _, h2) = R.forward(p.astype(np.float64), x.astype(np.float64))
+    # relative error <= 1e-5, relative to the magnitude of the terms each
+    # output sums (sum |W3 h2| + |b3|), so outputs that cancel to near zero
+    # are held to the same float32 rounding as the others
+    w = R.unpack(p.astype(np.float64))
+    scale = np.abs(h2) @ np.abs(w["W3"]).T + np.abs(w["b3"])
+    assert (np.abs(y - want) <= 1e-5 * scale).all(), np.max(np.abs(y - want) / scale)
 
 
 def test_mlp_gradient_matches_reference():
