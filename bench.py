@@ -387,10 +387,16 @@ def cpu_baseline_run(O, view, cgr, deg, w, K, budget_s=20.0):
                 "sample": f"{reps} full synchronous oracle steps over all {cgr.n} vertices ({arcs} arcs, "
                           f"K = {K}), {el:.2f} s each"}
     n_take = int(min(cgr.n, max(n_probe, n_probe * budget_s / max(probe_s, 1e-3))))
-    sample = np.sort(pos_all[:n_take])
-    t = time.perf_counter()
-    O.step_cgr_subset(view, lab, sample, out, c, seeds)
-    el = time.perf_counter() - t
+    for attempt in range(2):
+        sample = np.sort(pos_all[:n_take])
+        t = time.perf_counter()
+        O.step_cgr_subset(view, lab, sample, out, c, seeds)
+        el = time.perf_counter() - t
+        # the small probe overstates the time per vertex: when the sample
+        # ran well under the budget, take one sized by its own rate
+        if attempt or el >= 0.5 * budget_s or n_take >= cgr.n:
+            break
+        n_take = int(min(cgr.n, n_take * budget_s / max(el, 1e-3)))
     arcs = int(deg[cgr.vertices[sample.astype(np.int64)]].sum())
     return {"value": K * arcs / el, "unit": "edges/s", "cores": cores, "kind": "port", "K": K,
             "sample": f"oracle decode+LP step (K = {K}) over {n_take} of {cgr.n} random stream positions "
