@@ -116,3 +116,15 @@ def test_graph_capture_refused_on_host():
     with pytest.raises(L.LpaError):
         e.stream_residence(True, 4096)
     e.close()
+
+
+def test_rmat22_k8_sixteen_mib_batches():
+    """R-MAT scale 22 (128 M arcs, 240 MB of stream) in 16 MiB batches, K = 8,
+    CMS 4 x 1024 above degree 4096: fused and two-kernel steps"""
+    cgr = L.gen_rmat(22, 16, 0.57, 0.19, 0.19, seed=1, bfs=True).encode(0, 128)
+    cfg = dict(cms_thresh=4096, cms_depth=4, cms_width=1024, cms_seed=0)
+    for method in ("nova", "cglp"):
+        e = host_engine(cgr, 16 << 20, K=8, **cfg)
+        assert e.stream_batches()[0] >= 10
+        run_and_compare(cgr, 3, K=8, method=method, engine=e, **cfg)
+        e.close()
