@@ -888,30 +888,51 @@ int lpa_graph_encode_csr(lpa_engine *e, uint64_t n, const uint64_t *row, const u
             hipLaunchKernelGGL(k_encode_len, dim3((unsigned)((n_c + 255) / 256)), dim3(256), 0, e->stream, drow.p,
                                dcol.p, e->vid0.p, n_c, (uint32_t)n, len.p, e->counter.p, enc::LONG_ROW);
         // rows longer than LONG_ROW: one workgroup each, with scratch of
-        // d + 1 entries per row for every per-entry array
-        std::vector<uint32_t> lpos;
-        std::vector<uint64_t> lbase(1, 0);
-        for (uint64_t p = 0; p < n_c; p++)
-            if (deg(vert[p]) > enc::LONG_ROW) {
-                lpos.push_back((uint32_t)p);
-                lbase.push_back(lbase.back() + deg(vert[p]) + 1);
-            }
+        // d + 1 entries per row for every per-entry array (44 bytes each),
+        // in groups of rows whose scratch stays within 2^28 entries
+        // (LPA_LONG_SCRATCH overrides the limit: tests force several groups)
+        uint64_t LONG_SCRATCH = 1ull << 28;
+        if (const char *ls = std::getenv("LPA_LONG_SCRATCH")) LONG_SCRATCH = std::max<uint64_t>(1, std::strtoull(ls, nullptr, 10));
+        struct LongGroup {
+            std::vector<uint32_t> pos;
+            std::vector<uint64_t> base{0};
+        };
+        std::vector<LongGroup> lgroups;
+        for (uint64_t p = 0; p < n_c; p++) {
+            const uint64_t d = deg(vert[p]);
+            if (d <= enc::LONG_ROW) continue;
+            if (lgroups.empty() || (lgroups.back().base.back() + d + 1 > LONG_SCRATCH && !lgroups.back().pos.empty()))
+                lgroups.emplace_back();
+            lgroups.back().pos.push_back((uint32_t)p);
+            lgroups.back().base.push_back(lgroups.back().base.back() + d + 1);
+        }
         DevBuf<uint32_t> dlpos, lleft, llen, lrid, lnxt_i, lnxt_r, lseg_i, lseg_r;
         DevBuf<uint64_t> dlbase, lP_i, lP_r;
         DevBuf<enc::LongInfo> linfo;
         enc::LongScratch lsc{};
-        if (!lpos.empty()) {
-            const uint64_t tot = lbase.back();
-            dlpos.upload(lpos.data(), lpos.size(), e->stream);
-            dlbase.upload(lbase.data(), lbase.size(), e->stream);
+        // the scratch of group g: its rows' entries, segment ends and chains
+        auto long_group = [&](const LongGroup &G) {
+            dlpos.upload(G.pos.data(), G.pos.size(), e->stream);
+            dlbase.upload(G.base.data(), G.base.size(), e->stream);
+            lsc = enc::LongScratch{dlbase.p, lleft.p, llen.p, lrid.p, lnxt_i.p, lnxt_r.p, lseg_i.p, lseg_r.p,
+                                   lP_i.p, lP_r.p, linfo.p};
+            hipLaunchKernelGGL(k_encode_long_len, dim3((unsigned)G.pos.size()), dim3(enc::LONG_LANES), 0, e->stream,
+                               drow.p, dcol.p, e->vid0.p, dlpos.p, (uint32_t)n, lsc, len.p, e->counter.p);
+        };
+        if (!lgroups.empty()) {
+            uint64_t tot = 0, rows = 0;
+            for (const auto &G : lgroups) {
+                tot = std::max(tot, G.base.back());
+                rows = std::max<uint64_t>(rows, G.pos.size());
+            }
             for (DevBuf<uint32_t> *b : {&lleft, &llen, &lrid, &lnxt_i, &lnxt_r, &lseg_i, &lseg_r}) b->alloc(tot);
             lP_i.alloc(tot);
             lP_r.alloc(tot);
-            linfo.alloc(lpos.size());
-            lsc = enc::LongScratch{dlbase.p, lleft.p, llen.p, lrid.p, lnxt_i.p, lnxt_r.p, lseg_i.p, lseg_r.p,
-                                   lP_i.p, lP_r.p, linfo.p};
-            hipLaunchKernelGGL(k_encode_long_len, dim3((unsigned)lpos.size()), dim3(enc::LONG_LANES), 0, e->stream,
-                               drow.p, dcol.p, e->vid0.p, dlpos.p, (uint32_t)n, lsc, len.p, e->counter.p);
+            linfo.alloc(rows);
+            for (const auto &G : lgroups) {
+                long_group(G);
+                e->sync();   // the next group reuses the scratch
+            }
         }
         const uint64_t nb = std::max<uint64_t>(1, (n_c + SCAN_PER_BLOCK - 1) / SCAN_PER_BLOCK);
         DevBuf<uint64_t> sums;
@@ -936,9 +957,14 @@ int lpa_graph_encode_csr(lpa_engine *e, uint64_t n, const uint64_t *row, const u
         if (n_c)
             hipLaunchKernelGGL(k_encode_write, dim3((unsigned)((n_c + 255) / 256)), dim3(256), 0, e->stream, drow.p,
                                dcol.p, e->vid0.p, n_c, e->offs0.p, e->words0.p, enc::LONG_ROW);
-        if (!lpos.empty())
-            hipLaunchKernelGGL(k_encode_long_write, dim3((unsigned)lpos.size()), dim3(enc::LONG_LANES), 0, e->stream,
-                               e->vid0.p, dlpos.p, lsc, e->offs0.p, e->words0.p);
+        for (const auto &G : lgroups) {
+            // one group: its scratch is still in place; several: each is
+            // measured again before it is written (the same lengths result)
+            if (lgroups.size() > 1) long_group(G);
+            hipLaunchKernelGGL(k_encode_long_write, dim3((unsigned)G.pos.size()), dim3(enc::LONG_LANES), 0,
+                               e->stream, e->vid0.p, dlpos.p, lsc, e->offs0.p, e->words0.p);
+            if (lgroups.size() > 1) e->sync();
+        }
         HIPCHK(hipGetLastError());
         if (e->host_stream) move_stream(e, e->words0, true);
         finish_graph(e);

@@ -86,6 +86,18 @@ def test_generated_graphs(gen, T):
     e.close()
 
 
+def test_long_rows_in_several_scratch_groups(monkeypatch):
+    """the long rows' scratch limited to 200,000 entries: the five hub rows
+    are encoded in groups, each measured and written in turn"""
+    monkeypatch.setenv("LPA_LONG_SCRATCH", "200000")
+    g = _long_rows_graph()
+    cgr = g.encode(0, 128)
+    e = L.Engine(0)
+    e.encode_csr(g.row, g.col, 0)
+    check_same_stream(e, cgr)
+    e.close()
+
+
 def test_lp_on_device_encoded_graph():
     g = L.gen_rmat(14, 16, seed=25)
     cgr = g.encode(0, 128)
