@@ -129,8 +129,12 @@ struct HeavySet {
     DevBuf<uint32_t> hcnt;                   // CMS: own-label counters per (slot, vector of a batch)
     uint32_t ntasks = 0;
     DevBuf<DecodeTask> tasks;                // decode tasks (fixed per layout)
-    DevBuf<uint32_t> stash;                  // CMS, several vectors: label stash per resident workgroup
+    DevBuf<uint32_t> stash;                  // CMS, several vectors: label stash per resident workgroup;
+                                             // exact, many vectors: row stash per resident workgroup
     uint32_t stash_blocks = 0;
+    uint32_t rstash_small = 0;               // exact: grid of the one-workgroup kernel of degrees up
+                                             // to CHUNK with a row stash (0: none, a workgroup per vertex)
+    DevBuf<uint32_t> wq;                     // its work and exit counters
     bool tagged = false;                     // exact: generation-tagged tables (see TABLE_GENS)
     DevBuf<uint32_t> epoch;                  // exact: passes completed
     std::vector<uint32_t> task_pos;          // stream position of every decode task (ascending)
@@ -438,6 +442,19 @@ static uint32_t cms_local_kb(uint32_t K, size_t cells, size_t lds) {
     return (uint32_t)std::min<size_t>({(size_t)K, (size_t)HEAVY_KB_CMS_LOCAL, 2 * pairs});
 }
 
+// Row stashes for the exact one-workgroup kernel (degrees up to CHUNK) from
+// this many label vectors up (below it a row is a line or less and the
+// batches are few); LPA_ROW_STASH=0 turns them off.
+constexpr uint32_t ROW_STASH_MIN_K = 16;
+
+static bool row_stash_on(const lpa_engine *e) {
+    const char *env = std::getenv("LPA_ROW_STASH");
+    return e->K >= ROW_STASH_MIN_K && !(env && std::strcmp(env, "0") == 0);
+}
+
+// words of a row stash per neighbour: K rounded up to whole groups of four
+static uint64_t row_stash_k(const lpa_engine *e) { return (e->K + 3ull) / 4 * 4; }
+
 static size_t cms_local_lds(uint32_t kbl, size_t cells) {
     return (kbl == 1 ? 1 : (kbl + 1) / 2) * cells * sizeof(uint32_t);
 }
@@ -500,12 +517,19 @@ void lpa_engine::launch_heavy(HeavySet &hs, const StepArgs &a, bool as_csr, bool
             if (one) hipLaunchKernelGGL((k_heavy_local<1, 1, 4>), g, b, sk, stream, h, hs.local.p, hs.nlocal);
             else hipLaunchKernelGGL((k_heavy_local<1, HEAVY_KB_CMS_LOCAL, 4>), g, b, sk, stream, h, hs.local.p, hs.nlocal);
         } else {
-            // tables of up to 2*CHUNK (8192 ids: 4 per lane) or 4*CHUNK slots (8 per lane)
+            // tables of up to 2*CHUNK (8192 ids: 4 per lane) or 4*CHUNK slots (8 per lane);
+            // with a row stash, resident workgroups take the vertices one by one
             const size_t t1 = 2 * CHUNK * sizeof(uint64_t), t2 = 4 * CHUNK * sizeof(uint64_t);
             if (nsmall) {
-                if (one) hipLaunchKernelGGL((k_heavy_local<0, 1, 4>), dim3(nsmall), b, t1, stream, h, lsmall, nsmall);
-                else hipLaunchKernelGGL((k_heavy_local<0, HEAVY_KB_LOCAL, 4>), dim3(nsmall), b, t1, stream, h, lsmall, nsmall);
+                h.stash = hs.rstash_small ? hs.stash.p : nullptr;
+                h.stash_stride = (uint64_t)CHUNK * row_stash_k(this);
+                h.wq = hs.wq.p;
+                const dim3 g(hs.rstash_small ? hs.rstash_small : nsmall);
+                if (one) hipLaunchKernelGGL((k_heavy_local<0, 1, 4>), g, b, t1, stream, h, lsmall, nsmall);
+                else if (h.stash) hipLaunchKernelGGL((k_heavy_local<0, HEAVY_KB_LOCAL, 4, true>), g, b, t1, stream, h, lsmall, nsmall);
+                else hipLaunchKernelGGL((k_heavy_local<0, HEAVY_KB_LOCAL, 4>), g, b, t1, stream, h, lsmall, nsmall);
             }
+            h.stash = nullptr;
             if (hs.nlocal_big) {
                 const dim3 g(hs.nlocal_big);
                 if (one) hipLaunchKernelGGL((k_heavy_local<0, 1, 8>), g, b, t2, stream, h, lbig, hs.nlocal_big);
@@ -738,6 +762,7 @@ int lpa_engine_create(int device, lpa_engine **out) {
                                        (const void *)k_heavy_chunks<1, 1>, (const void *)k_heavy_chunks<1, HEAVY_KB_CMS_LOCAL>,
                                        (const void *)k_heavy_local<0, 1, 4>, (const void *)k_heavy_local<0, HEAVY_KB_LOCAL, 4>,
                                        (const void *)k_heavy_local<0, 1, 8>, (const void *)k_heavy_local<0, HEAVY_KB_MAX, 8>,
+                                       (const void *)k_heavy_local<0, HEAVY_KB_LOCAL, 4, true>,
                                        (const void *)k_heavy_local<1, 1, 4>, (const void *)k_heavy_local<1, HEAVY_KB_CMS_LOCAL, 4>};
             for (const void *f : heavy_fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
             (void)hipGetLastError();
@@ -1260,6 +1285,21 @@ static void build_heavy(lpa_engine *e) {
                 HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, HBLOCK, cms_local_lds(kbl, cells)));
                 hs->stash_blocks = (uint32_t)std::min<uint64_t>(hs->nlocal, (uint64_t)std::max(per_cu, 1) * e->num_cu);
                 hs->stash.alloc((uint64_t)hs->stash_blocks * CMS_LOCAL * (one ? 1 : HEAVY_KB_CMS_LOCAL));
+            }
+            const uint32_t nsmall = hs->nlocal - hs->nlocal_big;
+            if (!hs->cms && nsmall && row_stash_on(e)) {
+                // exact one-workgroup vertices of degree <= CHUNK with many
+                // vectors: a row stash (CHUNK x K words) per workgroup the CUs
+                // hold at once.  (Degrees up to 2*CHUNK keep a workgroup per
+                // vertex: their stashes, 2 MB each at K = 64, overflow the
+                // caches and the step measured slower with them.)
+                int occ = 0;
+                HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &occ, (const void *)k_heavy_local<0, HEAVY_KB_LOCAL, 4, true>, HBLOCK, 2 * CHUNK * sizeof(uint64_t)));
+                hs->rstash_small = (uint32_t)std::min<uint64_t>(nsmall, (uint64_t)std::max(occ, 1) * e->num_cu);
+                hs->stash.alloc((uint64_t)hs->rstash_small * CHUNK * row_stash_k(e));
+                hs->wq.alloc(2);
+                HIPCHK(hipMemsetAsync(hs->wq.p, 0, 2 * sizeof(uint32_t), e->stream));
             }
             e->heavy.push_back(std::move(hs));
         }

@@ -57,11 +57,14 @@ struct HeavyArgs {
     uint32_t kbs;                              // CMS: label vectors per sketch pass (sketches side by side in LDS)
     uint32_t kbl;                              // CMS one-workgroup kernel: vectors per pass (packed 16-bit cells)
     const uint32_t *__restrict__ slot;         // chunked vertex i -> its sketch / result slot
-    uint32_t *stash;                           // CMS, several vectors: per-workgroup label stash (or NULL)
+    uint32_t *stash;                           // CMS, several vectors: per-workgroup label stash; exact
+                                               // one-workgroup kernel: per-workgroup row stash (or NULL)
     uint64_t stash_stride;                     // words per workgroup
     uint32_t *epoch;                           // exact mode, tagged tables: passes completed (device word)
     int tagged;                                // exact mode: HBM table slots carry the pass's generation
     uint32_t kb_alloc;                         // exact mode: vectors the table buffer is laid out for
+    uint32_t *wq;                              // exact one-workgroup kernel with a row stash: work
+                                               // counter and exit counter (zero between launches)
     uint32_t *hcnt;                            // CMS, chunked hubs: per (slot, vector of the batch) 4 words:
                                                // own-label count, labelled count, full-collision flag, unused
 };
@@ -717,7 +720,37 @@ __device__ __forceinline__ void cms_local_own(const StepArgs &a, const uint32_t 
     __syncthreads();   // the sketches, cursors and counts are reused next
 }
 
-template <int CMS, int KB, int IPL>
+// Row stash of an exact one-workgroup vertex: the clean labels of all K
+// vectors of neighbours ids[0..n) in groups of four vectors (vector k of
+// neighbour i at st[((k/4)*n + i)*4 + k%4]; ceil(K/4)*4*n words).  Each
+// neighbour's label row is read whole, once (its two 128-byte lines at
+// K = 64), instead of once per batch of KB vectors, where every batch
+// fetches the lines again for a KB-label slice; the batches then read the
+// stash with coalesced 16-byte loads.  Lane i % HBLOCK writes neighbour i,
+// the lane that count_staged reads it in.
+__device__ __forceinline__ void stash_rows(const StepArgs &a, const uint32_t *__restrict__ ids, uint32_t n,
+                                           uint32_t *__restrict__ st) {
+    const uint32_t K = a.K;
+    uint4 *st4 = reinterpret_cast<uint4 *>(st);
+    for (uint32_t i = threadIdx.x; i < n; i += HBLOCK) {
+        const uint32_t *src = a.lin + (uint64_t)ids[i] * K;
+        if ((K & 15) == 0) {
+            for (uint32_t k = 0; k < K; k += 16) {
+                uint4 w[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) w[q] = reinterpret_cast<const uint4 *>(src + k)[q];
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    st4[(uint64_t)(k / 4 + q) * n + i] = make_uint4(clean_label(w[q].x), clean_label(w[q].y),
+                                                                    clean_label(w[q].z), clean_label(w[q].w));
+            }
+        } else {
+            for (uint32_t k = 0; k < K; k++) st[((uint64_t)(k >> 2) * n + i) * 4 + (k & 3)] = clean_label(src[k]);
+        }
+    }
+}
+
+template <int CMS, int KB, int IPL, bool ST>
 __device__ __forceinline__ void heavy_local_vertex(const HeavyArgs &h, uint32_t i, uint64_t *lds, uint64_t *red,
                                                    uint32_t *olds, uint32_t *xw, uint32_t *xs, CmsLocalShared *cs) {
     static_assert(KB <= HEAVY_KB_MAX, "olds[] holds one batch");
@@ -756,6 +789,12 @@ __device__ __forceinline__ void heavy_local_vertex(const HeavyArgs &h, uint32_t 
         const uint32_t T = d ? 1u << (32 - __builtin_clz(2 * d - 1)) : 1u;
         const uint32_t logT = 31 - __builtin_clz(T);
         uint64_t *tab = lds;
+        // with a row stash (many vectors) every row is read once for all K
+        uint32_t *st = ST ? h.stash + (uint64_t)blockIdx.x * h.stash_stride : nullptr;
+        if (ST) {
+            __syncthreads();   // the previous vertex's reads of the region are done
+            stash_rows(a, ids, d, st);
+        }
         for (uint32_t k0 = 0; k0 < a.K; k0 += KB) {
             const uint32_t kb = a.K - k0 < (uint32_t)KB ? a.K - k0 : (uint32_t)KB;
             uint64_t best = 0;
@@ -789,7 +828,8 @@ __device__ __forceinline__ void heavy_local_vertex(const HeavyArgs &h, uint32_t 
                 },
                 [&](int s, uint64_t key) {
                     if (threadIdx.x == 0) finish(k0, s, key);
-                });
+                },
+                ST ? st + (uint64_t)k0 * d : nullptr, d);   // k0 is a multiple of 4
         }
     }
     if (threadIdx.x == 0 && nch) atomicAdd(a.changed, (unsigned long long)nch);
@@ -805,21 +845,45 @@ __device__ __forceinline__ void heavy_local_vertex(const HeavyArgs &h, uint32_t 
 // (Eight waves per SIMD requested -- two 1024-lane workgroups per CU -- when
 // the tables leave LDS for two: IPL = 4; the 128 KB tables of IPL = 8 admit
 // one workgroup per CU, so those instances take four waves per SIMD and the
-// registers that frees.)
-template <int CMS, int KB, int IPL>
+// registers that frees.)  ST (exact mode, IPL = 4): with a row stash, the
+// grid is the workgroups the CUs hold at once, each taking vertices one by
+// one into its own stash region (h.stash); without one, a workgroup per
+// vertex.
+template <int CMS, int KB, int IPL, bool ST = false>
 __global__ void __launch_bounds__(HBLOCK, IPL == 8 || (CMS && KB > 1) ? 4 : 8)
     k_heavy_local(HeavyArgs h, const uint32_t *__restrict__ local, uint32_t nlocal) {
+    static_assert(!ST || (!CMS && IPL * HBLOCK == CHUNK), "row stash regions hold CHUNK neighbours");
     extern __shared__ uint64_t lds[];
     __shared__ uint64_t red[HBLOCK / 64];
     __shared__ uint32_t olds[HEAVY_KB_MAX];
     __shared__ uint32_t xw[CMS ? 1 : HEAVY_KB_MAX * (HBLOCK / 64)], xs[HEAVY_KB_MAX];   // block_sum_words
     __shared__ CmsLocalShared cs[1];
-    if (CMS) {
+    if (ST) {
+        // workgroups take the next vertex from the work counter h.wq[0]
+        // (vertices of very different degrees: a fixed share per workgroup
+        // leaves some with far more work); the last workgroup to leave
+        // resets the counters for the next launch, when no workgroup of
+        // this one reads them any more
+        __shared__ uint32_t next;
+        for (;;) {
+            if (threadIdx.x == 0) next = atomicAdd(&h.wq[0], 1u);
+            __syncthreads();
+            const uint32_t j = next;
+            if (j >= nlocal) break;
+            // the vertex synchronises the workgroup before thread 0 writes
+            // `next` again
+            heavy_local_vertex<CMS, KB, IPL, ST>(h, local[j], lds, red, olds, xw, xs, cs);
+        }
+        if (threadIdx.x == 0 && atomicAdd(&h.wq[1], 1u) == gridDim.x - 1) {
+            atomicExch(&h.wq[0], 0u);
+            atomicExch(&h.wq[1], 0u);
+        }
+    } else if (CMS) {
         // workgroups loop over the vertices, each with its stash region
         for (uint32_t j = blockIdx.x; j < nlocal; j += gridDim.x)
-            heavy_local_vertex<CMS, KB, IPL>(h, local[j], lds, red, olds, xw, xs, cs);
+            heavy_local_vertex<CMS, KB, IPL, ST>(h, local[j], lds, red, olds, xw, xs, cs);
     } else if (blockIdx.x < nlocal) {
-        heavy_local_vertex<CMS, KB, IPL>(h, local[blockIdx.x], lds, red, olds, xw, xs, cs);
+        heavy_local_vertex<CMS, KB, IPL, ST>(h, local[blockIdx.x], lds, red, olds, xw, xs, cs);
     }
 }
 

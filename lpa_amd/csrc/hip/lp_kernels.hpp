@@ -542,17 +542,32 @@ __device__ __forceinline__ uint32_t own_majority(const uint32_t (&labs)[IPL][KB]
 // no table work; for every other vector s: begin(s) prepares the table (and
 // synchronises the group); staged labels go through aggregated rounds
 // (agg(s, lab, count) by one lane per match) and then use(s, lab) one at a
-// time; end(s) finishes the vector.
+// time; end(s) finishes the vector.  st (or null, KB a multiple of 4): the
+// labels come from a row stash instead of the label rows -- clean labels
+// in groups of four vectors, vector k0+s of neighbour i at
+// st[((s/4)*sn + i)*4 + s%4] -- written by the lane that reads them (lane
+// i % G), so no synchronisation is needed.
 template <int G, int IPL, int KB, class Begin, class Agg, class Use, class End, class Decide>
 __device__ __forceinline__ void count_staged(const StepArgs &a, const uint32_t *__restrict__ ids, uint32_t d,
                                              uint32_t gl, uint32_t k0, uint32_t kb, const uint32_t (&own)[KB],
                                              bool use_own, uint32_t *xw, uint32_t *xs, Begin &&begin, Agg &&agg,
-                                             Use &&use, End &&end, Decide &&decide) {
+                                             Use &&use, End &&end, Decide &&decide,
+                                             const uint32_t *__restrict__ st = nullptr, uint32_t sn = 0) {
     uint32_t labs[IPL][KB];
 #pragma unroll
     for (int t = 0; t < IPL; t++) {
         const uint32_t i = t * G + gl;
-        if (i < d) {
+        if (KB % 4 == 0 && i < d && st) {
+#pragma unroll
+            for (int q = 0; q < KB / 4; q++) {
+                const uint4 w = 4u * q < kb ? reinterpret_cast<const uint4 *>(st)[(uint64_t)q * sn + i]
+                                            : make_uint4(UNL, UNL, UNL, UNL);
+                labs[t][4 * q] = w.x;
+                labs[t][4 * q + 1] = 4u * q + 1 < kb ? w.y : UNL;
+                labs[t][4 * q + 2] = 4u * q + 2 < kb ? w.z : UNL;
+                labs[t][4 * q + 3] = 4u * q + 3 < kb ? w.w : UNL;
+            }
+        } else if (i < d) {
             load_row<KB>(a, ids[i], k0, kb, labs[t]);
         } else {
 #pragma unroll
@@ -577,7 +592,8 @@ __device__ __forceinline__ void count_staged(const StepArgs &a, const uint32_t *
             for (int t = 0; t < IPL; t++)
                 if (pm & (1u << t)) use(s, labs[t][s]);
             for (uint32_t i = IPL * G + gl; i < d; i += G) {
-                const uint32_t lab = clean_label(a.lin[(uint64_t)ids[i] * a.K + k0 + s]);
+                const uint32_t lab = KB % 4 == 0 && st ? st[((uint64_t)(s >> 2) * sn + i) * 4 + (s & 3)]
+                                                       : clean_label(a.lin[(uint64_t)ids[i] * a.K + k0 + s]);
                 if (lab != UNL) use(s, lab);
             }
             end(s);
