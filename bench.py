@@ -231,7 +231,11 @@ def main():
             f"{'learned' if kept else 'degree'} kept")
         layout = "learned" if kept else "degree"
         if w.get("autotune") and not kept:
-            # the degree layout stays: its bin thresholds from the autotuner
+            # the degree layout stays: its bin thresholds from the autotuner,
+            # tuned on the steps the headline times (after the warm-up)
+            e.labels_init(labK)
+            for _ in range(args.warmup):
+                e.nova_iterate()
             tuned, tuned_ms = e.autotune(iters=2)
             res["autotune"] = dict(params=tuned, median_ms=tuned_ms)
             layout = "degree (autotuned)"

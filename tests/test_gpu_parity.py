@@ -134,18 +134,23 @@ def test_graph_replay_matches_eager():
         assert np.array_equal(eager.labels(), graph.labels()), it
 
 
-@pytest.mark.parametrize("K,cms", [(1, False), (9, False), (8, True)])
-def test_graph_replay_hubs_across_generation_wrap(K, cms):
+@pytest.mark.parametrize("K,cms,table_gb", [(1, False, None), (9, False, None), (8, True, None),
+                                            (9, False, "0.004")])
+def test_graph_replay_hubs_across_generation_wrap(K, cms, table_gb, monkeypatch):
     """steps replayed from captured HIP graphs on the hub graph (the
     one-workgroup and chunked heavy kernels: generation-tagged exact tables,
     count-min sketches and the own-label counters) match eager steps, from
     few-valued random labels, across the exact tables' generation wrap (the
-    pass counter reaches 255 after 250 warm-up steps)"""
+    pass counter reaches 255 after 250 warm-up steps; with a 4 MiB table
+    budget the hub's table holds 2 of 9 vectors, five passes a step)"""
     cgr = _hub_graph().encode(0, 128)
     cfg = dict(cms_thresh=2500, cms_depth=4, cms_width=1024, cms_seed=4) if cms else {}
-    print(f"[gpu-case] replay hubs K={K} cms={cms}", flush=True)
+    print(f"[gpu-case] replay hubs K={K} cms={cms} table_gb={table_gb}", flush=True)
+    if table_gb:
+        monkeypatch.setenv("LPA_TABLE_GB", table_gb)
     eager = make_engine(cgr, K=K, **cfg)
     graph = make_engine(cgr, K=K, **cfg)
+    monkeypatch.delenv("LPA_TABLE_GB", raising=False)
     eager.labels_init()
     graph.labels_init()
     for _ in range(250):
@@ -255,6 +260,33 @@ def test_exact_tables_across_generation_wrap():
     rng = np.random.default_rng(5)
     for it in range(8):
         lab = rng.integers(0, 40, cgr.n).astype(np.uint32)
+        e.labels_init(lab)
+        ch = e.nova_iterate()
+        exp, och = O.step_cgr(view, lab, c, None)
+        assert np.array_equal(e.labels(), exp), it
+        assert ch == och, it
+    e.close()
+
+
+def test_exact_tables_in_batches(monkeypatch):
+    """a 4 MiB table budget (LPA_TABLE_GB) holds the 70,000-id hub's table
+    for 2 of 9 vectors: five batches a step, each gathering the hub's rows
+    again and advancing the tables' generation, which so wraps every 51
+    steps; after 48 steps, eight steps from fresh random labels straddle a
+    wrap, each against the oracle"""
+    monkeypatch.setenv("LPA_TABLE_GB", "0.004")
+    cgr = _hub_graph().encode(0, 128)
+    K = 9
+    e = make_engine(cgr, K=K)
+    monkeypatch.delenv("LPA_TABLE_GB")
+    e.labels_init()
+    for _ in range(48):
+        e.nova_iterate()
+    view = oracle_view(cgr)
+    c = O.cfg(K=K)
+    rng = np.random.default_rng(6)
+    for it in range(8):
+        lab = rng.integers(0, 40, cgr.n * K).astype(np.uint32)
         e.labels_init(lab)
         ch = e.nova_iterate()
         exp, och = O.step_cgr(view, lab, c, None)
