@@ -30,6 +30,23 @@ def test_each_kernel_family(sched):
     run_and_compare(g.encode(0, 128), 4, **sched)
 
 
+@pytest.mark.parametrize("K", [16, 32])
+@pytest.mark.parametrize("sched", [dict(lane_max=0, wave_max=512), dict(lane_max=8, wave_max=128, quarter_max=64),
+                                   dict(lane_max=8, wave_max=32, block_max=64)])
+def test_group_kernels_several_batches(K, sched):
+    """quarter-wave, half-wave, wave and workgroup groups counting two
+    (K = 16) or four (K = 32) batches of 8 vectors per vertex"""
+    g = L.gen_rmat(13, 16, seed=11)
+    run_and_compare(g.encode(0, 128), 3, K=K, **sched)
+
+
+def test_group_kernels_several_batches_csr_part():
+    """the same over the small-vertex CSR part (T = 32) of a copying-model
+    graph, 32 vectors"""
+    g = L.gen_copying(20000, 30.0, 2.1, 3000, 0.7, 200, seed=2)
+    run_and_compare(g.encode(32, 128), 3, K=32)
+
+
 def test_device_degrees_and_decode_to_csr():
     g = L.gen_rmat(14, 16, seed=12)
     cgr = g.encode(0, 128)
