@@ -152,14 +152,16 @@ def test_graph_replay_matches_eager():
 
 
 @pytest.mark.parametrize("K,cms,table_gb", [(1, False, None), (9, False, None), (8, True, None),
-                                            (9, False, "0.004")])
+                                            (9, False, "0.004"), (64, False, None)])
 def test_graph_replay_hubs_across_generation_wrap(K, cms, table_gb, monkeypatch):
     """steps replayed from captured HIP graphs on the hub graph (the
     one-workgroup and chunked heavy kernels: generation-tagged exact tables,
     count-min sketches and the own-label counters) match eager steps, from
     few-valued random labels, across the exact tables' generation wrap (the
     pass counter reaches 255 after 250 warm-up steps; with a 4 MiB table
-    budget the hub's table holds 2 of 9 vectors, five passes a step)"""
+    budget the hub's table holds 2 of 9 vectors, five passes a step; at
+    K = 64 the degree-3,000 hub goes through the row-stash kernel, whose
+    work counter is reset by its own last workgroup on every replay)"""
     cgr = _hub_graph().encode(0, 128)
     cfg = dict(cms_thresh=2500, cms_depth=4, cms_width=1024, cms_seed=4) if cms else {}
     print(f"[gpu-case] replay hubs K={K} cms={cms} table_gb={table_gb}", flush=True)
